@@ -1,0 +1,42 @@
+"""Float64 restatement of the reference FFT convolution (test infrastructure only).
+
+Follows pymoresane/iuwt_convolution.py:18-73 (cpu branch of ``fft_convolve``)
+and 142-158 (``pad_array``), and the PSF-spectrum precompute of
+pymoresane/main.py:175-213.
+
+* linear:   ``fftshift(irfft2(S * rfft2(pad(x))))[N/2:3N/2, N/2:3N/2]`` where
+  ``S = rfft2(psf_2N)``; closed form ``fftconvolve(x, psf_2N)[N:2N, N:2N]``.
+* circular: ``fftshift(irfft2(S * rfft2(x)))`` with ``S = rfft2(psf_N)``;
+  closed form ``out[i] = sum_u x[u] psf[(i - u + N/2) mod N]``.
+
+Everything is complex128/float64 here (the reference returns float32 for the
+linear branch, iuwt_convolution.py:70; the oracle keeps float64).
+"""
+
+import numpy as np
+
+
+def pad_array(in1):
+    """Zero frame of twice the size with ``in1`` in the centre (iuwt_convolution.py:142-158)."""
+    in1 = np.asarray(in1, dtype=np.float64)
+    h, w = in1.shape
+    out = np.zeros((2 * h, 2 * w))
+    out[h // 2:h // 2 + h, w // 2:w // 2 + w] = in1
+    return out
+
+
+def psf_spectrum(psf_frame):
+    """rfft2 of a PSF frame (main.py:178-213), complex128."""
+    return np.fft.rfft2(np.asarray(psf_frame, dtype=np.float64))
+
+
+def fft_convolve(in1, in2, conv_mode="linear"):
+    """Convolve ``in1`` with the PSF whose rfft2 is ``in2`` (iuwt_convolution.py:62-73)."""
+    in1 = np.asarray(in1, dtype=np.float64)
+    if conv_mode == "linear":
+        h, w = in1.shape
+        full = np.fft.fftshift(np.fft.irfft2(in2 * np.fft.rfft2(pad_array(in1))))
+        return full[h // 2:h // 2 + h, w // 2:w // 2 + w]
+    if conv_mode == "circular":
+        return np.fft.fftshift(np.fft.irfft2(in2 * np.fft.rfft2(in1)))
+    raise ValueError("conv_mode must be 'linear' or 'circular', got %r" % (conv_mode,))
