@@ -1,0 +1,577 @@
+// Device-resident MORESANE driver (major loop, conjugate-gradient minor loop,
+// by-scale outer loop) for one job on one HIP stream.
+//
+// Control flow follows pymoresane/main.py:53-599 step for step; every array
+// stays in HBM and only the per-iteration decision scalars (normalised scale
+// maxima, the SNR sums, the residual standard deviation) are read back:
+//   * major iteration (main.py:278-300): decomposition of the dirty
+//     subregion, exact MAD thresholds, thresholding with the edge mask, and
+//     per-scale maxima divided by the PSF scale energies;
+//   * scale choice (main.py:304-340) on the host from those maxima;
+//   * source extraction + recomposition (main.py:347-361) on the device;
+//   * CG minor loop (main.py:365-457): vectors in float32, every inner
+//     product, step size and norm in float64; the positivity clip branch is
+//     predicated on a device flag so no extra synchronisation happens;
+//     one host sync per iteration reads the SNR sums for the exit rules;
+//   * model update and residual (main.py:479-506), with revert.
+// Kernels run with a null predicate here; the predicate mechanism exists so a
+// caller can enqueue work whose execution depends on device state.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "engine.h"
+
+namespace pms {
+namespace {
+
+constexpr int kT = 256;
+
+__global__ void k_div(const double* a, const double* b, double* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *out = *a / *b;
+}
+
+// xn = x + alpha p ; partial min of xn
+__global__ __launch_bounds__(kT) void k_cg_xn(const float* __restrict__ x, const float* __restrict__ p,
+                                              const double* __restrict__ alpha, float* __restrict__ xn, size_t n,
+                                              float* __restrict__ pmin) {
+  const double a = *alpha;
+  float m = INFINITY;
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) {
+    const float v = (float)fma(a, (double)p[i], (double)x[i]);
+    xn[i] = v;
+    m = fminf(m, v);
+  }
+  m = block_min_f32(m);
+  if (threadIdx.x == 0) pmin[blockIdx.x] = m;
+}
+
+__global__ void k_cg_clipflag(const float* pmin, int nb, int enforce, int* flag, float* xmin) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  float m = INFINITY;
+  for (int b = 0; b < nb; ++b) m = fminf(m, pmin[b]);
+  *xmin = m;
+  *flag = (enforce && m < 0.f) ? 1 : 0;
+}
+
+// positivity clip: xn[xn<0] = 0 ; p = (xn - x) / alpha
+__global__ __launch_bounds__(kT) void k_cg_clip(float* __restrict__ xn, const float* __restrict__ x,
+                                                float* __restrict__ p, const double* __restrict__ alpha, size_t n,
+                                                const int* flag) {
+  if (*flag == 0) return;
+  const double a = *alpha;
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) {
+    float v = xn[i];
+    if (v < 0.f) v = 0.f;
+    xn[i] = v;
+    p[i] = (float)(((double)v - (double)x[i]) / a);
+  }
+}
+
+// rn = r - alpha Ap ; partial sum rn^2
+__global__ __launch_bounds__(kT) void k_cg_rn(const float* __restrict__ r, const float* __restrict__ ap,
+                                              const double* __restrict__ alpha, float* __restrict__ rn, size_t n,
+                                              double* __restrict__ partial) {
+  const double a = *alpha;
+  double acc = 0.0;
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) {
+    const float v = (float)fma(-a, (double)ap[i], (double)r[i]);
+    rn[i] = v;
+    acc = fma((double)v, (double)v, acc);
+  }
+  acc = block_sum_f64(acc);
+  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+// p = rn + beta p
+__global__ __launch_bounds__(kT) void k_cg_p(const float* __restrict__ rn, float* __restrict__ p,
+                                             const double* __restrict__ beta, size_t n) {
+  const double b = *beta;
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT)
+    p[i] = (float)fma(b, (double)p[i], (double)rn[i]);
+}
+
+// model[r0+r, c0+c] += gain * x[r, c]
+__global__ __launch_bounds__(kT) void k_model_add(float* __restrict__ model, int ld, int r0, int c0, int n,
+                                                  const float* __restrict__ x, double gain) {
+  const size_t tot = (size_t)n * n;
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < tot; i += (size_t)gridDim.x * kT) {
+    const int r = (int)(i / n), c = (int)(i % n);
+    float* m = model + (size_t)(r0 + r) * ld + (c0 + c);
+    *m = *m + (float)(gain * (double)x[i]);
+  }
+}
+
+// out = a - b
+__global__ __launch_bounds__(kT) void k_sub(const float* __restrict__ a, const float* __restrict__ b,
+                                            float* __restrict__ out, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) out[i] = a[i] - b[i];
+}
+
+// plane += other
+__global__ __launch_bounds__(kT) void k_add_inplace(float* __restrict__ a, const float* __restrict__ b, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) a[i] += b[i];
+}
+
+int grid_for(size_t n) { return reduce_grid(n); }
+
+pms_trace_event ev(int kind, int major, int a, int b, int c, double x, double y) {
+  pms_trace_event e{};
+  e.kind = kind;
+  e.major = major;
+  e.a = a;
+  e.b = b;
+  e.c = c;
+  e.x = x;
+  e.y = y;
+  return e;
+}
+
+}  // namespace
+
+// --------------------------------------------------------------------------- buffers
+DeviceArena::~DeviceArena() { release(); }
+
+void DeviceArena::release() {
+  for (void* p : ptrs_) (void)hipFree(p);
+  ptrs_.clear();
+  bytes_ = 0;
+}
+
+void* DeviceArena::alloc(size_t bytes) {
+  void* p = nullptr;
+  const size_t b = ((bytes + 255) / 256) * 256;
+  if (b == 0) return nullptr;
+  hipError_t e = hipMalloc(&p, b);
+  if (e != hipSuccess) fail(PMS_ERR_ALLOC, "device allocation of " + std::to_string(b) + " bytes failed");
+  ptrs_.push_back(p);
+  bytes_ += b;
+  return p;
+}
+
+// --------------------------------------------------------------------------- geometry
+Geometry make_geometry(int n, int psf_n, const MoresaneParams& prm, int scale_count_in, int* subregion_out) {
+  Geometry g{};
+  PMS_REQUIRE(n >= 4 && (n % 2) == 0, "Image size is uneven. Please use even dimensions.");
+  int sub = prm.subregion;
+  if (sub <= 0 || sub > n) sub = n;
+  PMS_REQUIRE(sub % 2 == 0 && sub >= 4, "subregion must be even and >= 4");
+  const double l2 = std::log2((double)n) - 1.0;
+  int S = scale_count_in;
+  if (S <= 0 || S > l2) S = (int)l2;
+  PMS_REQUIRE(S >= 1, "image too small for one wavelet scale");
+  g.n = n;
+  g.psf_n = psf_n;
+  g.sub = sub;
+  g.S = S;
+  g.lo = n / 2 - sub / 2;
+  g.linear = prm.conv_mode == 0;
+  g.double_psf = psf_n == 2 * n;
+  PMS_REQUIRE(g.double_psf || psf_n == n, "PSF must be the dirty image size or twice it");
+  if (g.linear) {
+    g.p_sub = 2 * sub;
+    g.p_full = 2 * n;
+  } else {
+    g.p_sub = sub;
+    g.p_full = n;
+  }
+  PMS_REQUIRE(is_pow2(g.p_sub) && is_pow2(g.p_full), "FFT convolution needs power-of-two image sides");
+  check_iuwt_scale(sub, sub, S - 1);
+  if (subregion_out) *subregion_out = sub;
+  return g;
+}
+
+// --------------------------------------------------------------------------- job
+Job::Job(hipStream_t st) : st_(st) {}
+Job::~Job() {}
+
+void Job::ensure(const Geometry& g) {
+  if (have_ && g.n == geo_.n && g.psf_n == geo_.psf_n && g.sub == geo_.sub && g.S <= cap_S_ &&
+      g.p_sub == geo_.p_sub && g.p_full == geo_.p_full) {
+    const int S_keep = cap_S_;
+    geo_ = g;
+    cap_S_ = S_keep;
+    return;
+  }
+  arena_.release();
+  geo_ = g;
+  cap_S_ = std::max(g.S, (int)std::log2((double)g.n) - 1);
+  const int S = cap_S_;
+  const size_t N2 = (size_t)g.n * g.n;
+  const size_t s2 = (size_t)g.sub * g.sub;
+  auto F = [&](size_t n) { return (float*)arena_.alloc(n * sizeof(float)); };
+  dirty_sub = F(s2);
+  psf_sub = F(s2);
+  D = F(S * s2);
+  Dcg = F(S * s2);
+  E = F(S * s2);
+  M = (uint8_t*)arena_.alloc(S * s2);
+  rec = F(s2);
+  x = F(s2);
+  xn = F(s2);
+  r = F(s2);
+  rn = F(s2);
+  p = F(s2);
+  ap = F(s2);
+  cbuf = F(s2);
+  iuwt_work = F(2 * s2);
+  model_local = F(N2);
+  resid = F(N2);
+  conv_full = F(N2);
+  spec_sub = (float2*)arena_.alloc(fft_spectrum_elems(g.p_sub) * sizeof(float2));
+  spec_full = (g.p_full == g.p_sub && g.sub == g.n) ? spec_sub
+                                                     : (float2*)arena_.alloc(fft_spectrum_elems(g.p_full) * sizeof(float2));
+  const size_t fw = std::max({fft_conv_work_elems(g.sub, g.p_sub), fft_conv_work_elems(g.n, g.p_full),
+                              fft_spectrum_elems(g.p_sub), fft_spectrum_elems(g.p_full)});
+  fft_work = (float2*)arena_.alloc(fw * sizeof(float2));
+  psf_frame = F((size_t)g.psf_n * g.psf_n);
+  mad_work = arena_.alloc(mad_work_bytes(S));
+  mad_work_init(mad_work, S, st_);
+  ext_work = arena_.alloc(extract_work_bytes(S, g.sub, g.sub));
+  extract_work_init(ext_work, S, g.sub, g.sub, st_);
+  const long np = std::max<long>({(long)iuwt_decompose_partials(g.sub, g.sub, S, 0), (long)reduce_grid(N2) * 2,
+                                  (long)reduce_grid(s2) * S, 4096L});
+  partial = (double*)arena_.alloc(np * sizeof(double));
+  pmax = (float*)arena_.alloc((size_t)S * 2048 * sizeof(float));
+  scal = (double*)arena_.alloc(64 * sizeof(double));
+  iflag = (int*)arena_.alloc(64 * sizeof(int));
+  border_dev = iflag + 16;
+  have_ = true;
+}
+
+// psf_sub, PSF spectra and per-scale PSF energies (main.py:126-226)
+void Job::setup_psf(const float* psf_dev, std::vector<double>& energies) {
+  const Geometry& g = geo_;
+  const int pn = g.psf_n;
+  const int plo = g.double_psf ? pn / 2 - g.sub / 2 : g.lo;
+  window_copy(psf_dev, pn, plo, plo, g.sub, nullptr, psf_sub, st_);
+  // spectra: the frame the reference transforms, expressed as (source window, q) for fft_psf_spectrum
+  auto spectrum_of = [&](int lo, int q, int p, float2* dst) {
+    const float* src = psf_dev;
+    if (!(lo == 0 && q == pn)) {
+      window_copy(psf_dev, pn, lo, lo, q, nullptr, psf_frame, st_);
+      src = psf_frame;
+    }
+    fft_psf_spectrum(src, q, p, dst, fft_work, st_);
+  };
+  const bool full = g.sub == g.n;
+  if (g.linear) {
+    if (g.double_psf) {
+      if (full) {
+        spectrum_of(0, pn, g.p_full, spec_sub);
+      } else {
+        spectrum_of(pn / 2 - g.sub, 2 * g.sub, g.p_sub, spec_sub);
+        spectrum_of(0, pn, g.p_full, spec_full);
+      }
+    } else {
+      // N x N PSF zero-padded to 2N (pad_array): q = N inside a 2N frame
+      if (full) {
+        spectrum_of(0, pn, g.p_full, spec_sub);
+      } else {
+        spectrum_of(pn / 2 - g.sub, 2 * g.sub, g.p_sub, spec_sub);
+        spectrum_of(0, pn, g.p_full, spec_full);
+      }
+    }
+  } else {
+    if (g.double_psf) {
+      spectrum_of(pn / 2 - g.sub / 2, g.sub, g.p_sub, spec_sub);
+      if (spec_full != spec_sub) spectrum_of(pn / 2 - g.n / 2, g.n, g.p_full, spec_full);
+    } else {
+      spectrum_of(g.lo, g.sub, g.p_sub, spec_sub);
+      if (spec_full != spec_sub) spectrum_of(0, pn, g.p_full, spec_full);
+    }
+  }
+  // energies of the PSF decomposition
+  const size_t s2 = (size_t)g.sub * g.sub;
+  iuwt_decompose(psf_sub, g.sub, g.sub, g.S, 0, D, nullptr, iuwt_work, nullptr, nullptr, nullptr, nullptr, st_);
+  const int gr = reduce_grid(s2);
+  plane_sumsq_partials(D, g.S, s2, partial, nullptr, st_);
+  sum_partials(partial, gr, g.S, scal + 32, nullptr, st_);
+  std::vector<double> e2(g.S);
+  PMS_HIP(hipMemcpyAsync(e2.data(), scal + 32, sizeof(double) * g.S, hipMemcpyDeviceToHost, st_));
+  PMS_HIP(hipStreamSynchronize(st_));
+  energies.resize(g.S);
+  for (int i = 0; i < g.S; ++i) energies[i] = std::sqrt(e2[i]);
+  PMS_HIP(hipMemcpyAsync(scal + 32, energies.data(), sizeof(double) * g.S, hipMemcpyHostToDevice, st_));
+}
+
+// residual = dirty - H model (full image); returns std of residual[subregion]
+double Job::residual_and_std(const float* dirty_dev, const float* model_dev) {
+  const Geometry& g = geo_;
+  const size_t N2 = (size_t)g.n * g.n;
+  fft_conv(model_dev, g.n, g.p_full, spec_full, conv_full, fft_work, nullptr, st_);
+  hipLaunchKernelGGL(k_sub, dim3(grid_for(N2)), dim3(kT), 0, st_, dirty_dev, (const float*)conv_full, resid, N2);
+  PMS_LAUNCH_CHECK();
+  window_copy(resid, g.n, g.lo, g.lo, g.sub, nullptr, dirty_sub, st_);
+  const size_t s2 = (size_t)g.sub * g.sub;
+  mean_std(dirty_sub, s2, partial, scal + 56, scal + 57, st_);
+  double v[2];
+  PMS_HIP(hipMemcpyAsync(v, scal + 56, sizeof(double) * 2, hipMemcpyDeviceToHost, st_));
+  PMS_HIP(hipStreamSynchronize(st_));
+  return std::sqrt(v[1] / (double)s2);
+}
+
+static std::vector<int> suppression_borders(int S, int edge_suppression, int edge_offset) {
+  std::vector<int> b;
+  if (edge_suppression) {
+    int corruption = 0;
+    for (int i = 0; i < S; ++i) {
+      corruption += 2 * (1 << i);
+      b.push_back(edge_offset > corruption ? edge_offset : corruption);
+    }
+  } else if (edge_offset > 0) {
+    b.assign(S, edge_offset);
+  }
+  return b;
+}
+
+int Job::moresane(const MoresaneParams& prm, int scale_count, const float* dirty_dev, const float* psf_dev,
+                  const float* mask_dev, float* model_dev, float* residual_dev, int* complete, Trace& trace) {
+  int sub = 0;
+  Geometry g = make_geometry(prm.n, prm.psf_n, prm, scale_count, &sub);
+  ensure(g);
+  g = geo_;
+  const int S = g.S;
+  const size_t s2 = (size_t)g.sub * g.sub;
+  const size_t N2 = (size_t)g.n * g.n;
+  const int gr = reduce_grid(s2);
+  std::vector<double> energies;
+  setup_psf(psf_dev, energies);
+  NoiseMask nm = noise_mask(g.sub, g.sub, prm.edge_excl, prm.int_excl);
+  std::vector<int> borders = suppression_borders(S, prm.edge_suppression, prm.edge_offset);
+  if (!borders.empty())
+    PMS_HIP(hipMemcpyAsync(border_dev, borders.data(), sizeof(int) * S, hipMemcpyHostToDevice, st_));
+
+  // model (local to this call) starts at zero; dirty subregion window
+  fill_f32(model_local, N2, 0.f, st_);
+  window_copy(dirty_dev, g.n, g.lo, g.lo, g.sub, nullptr, dirty_sub, st_);
+  plane_max(dirty_sub, 1, s2, 0, pmax, nullptr, st_);
+  max_final(pmax, threshold_grid(s2), 1, (float*)(scal + 60), nullptr, nullptr, nullptr, st_);
+  float dmax = 0.f;
+  PMS_HIP(hipMemcpyAsync(&dmax, scal + 60, sizeof(float), hipMemcpyDeviceToHost, st_));
+  PMS_HIP(hipStreamSynchronize(st_));
+
+  double* thr = scal + 0;       // [0, 16)  thresholds (S <= 16)
+  double* nsm_dev = scal + 16;  // [16, 32) normalised maxima
+  double* energy_dev = scal + 32;
+  double* d_rr = scal + 48;
+  double* d_pap = scal + 49;
+  double* d_alpha = scal + 50;
+  double* d_rnrn = scal + 51;
+  double* d_beta = scal + 52;
+  double* d_d2 = scal + 53;
+  double* d_e2 = scal + 54;
+  int* clip = iflag + 0;
+  float* xmin = (float*)(iflag + 1);
+  PMS_REQUIRE(S <= 16, "scale_count above 16 is not supported");
+
+  int major = 0;
+  double max_coeff = 1.0;
+  double std_current = 1000.0, std_ratio = 1.0;
+  int min_scale = 0;
+  std::vector<double> nsm(S, 0.0);
+  const int pmx = threshold_grid(s2);
+
+  while (major < prm.major_loop_miter && max_coeff > 0 && std_ratio > prm.accuracy &&
+         (double)dmax > prm.flux_threshold) {
+    int max_scale = 0, scale_adjust = 0;
+    while (min_scale < S) {
+      if (min_scale == 0) {
+        iuwt_decompose(dirty_sub, g.sub, g.sub, S, 0, D, nullptr, iuwt_work, nullptr, nullptr, nullptr, nullptr, st_);
+        mad_thresholds(D, S, g.sub, g.sub, nm, thr, mad_work, nullptr, st_);
+        if (mask_dev) {
+          // decomposition of dirty_sub * mask[sub] replaces the thresholded one (main.py:283-285)
+          window_copy(mask_dev, g.n, g.lo, g.lo, g.sub, nullptr, cbuf, st_);
+          window_copy(dirty_sub, g.sub, 0, 0, g.sub, cbuf, rec, st_);
+          iuwt_decompose(rec, g.sub, g.sub, S, 0, D, nullptr, iuwt_work, nullptr, nullptr, nullptr, nullptr, st_);
+        }
+        apply_threshold(D, D, S, g.sub, g.sub, thr, prm.sigma_level, borders.empty() ? nullptr : border_dev, pmax,
+                        nullptr, st_);
+        max_final(pmax, pmx, S, nullptr, energy_dev, nsm_dev, nullptr, st_);
+        PMS_HIP(hipMemcpyAsync(nsm.data(), nsm_dev, sizeof(double) * S, hipMemcpyDeviceToHost, st_));
+        PMS_HIP(hipStreamSynchronize(st_));
+      }
+      int max_index = min_scale;
+      for (int i = min_scale + 1; i < S; ++i)
+        if (nsm[i] > nsm[max_index]) max_index = i;
+      max_scale = max_index + 1;
+      max_coeff = nsm[max_index];
+      if (max_coeff == 0) {
+        trace.push_back(ev(EV_NOCOEFF, major, 0, 0, 0, 0.0, 0.0));
+        break;
+      }
+      scale_adjust = 0;
+      for (int i = max_index - 1; i >= 0; --i)
+        if (nsm[i] == 0) {
+          scale_adjust = i + 1;
+          break;
+        }
+      trace.push_back(ev(EV_SCALES, major, min_scale, max_scale, scale_adjust, max_coeff, 0.0));
+      const int ns = max_scale - scale_adjust;
+      const float* tslice = D + (size_t)scale_adjust * s2;
+      // masks/extracted live at planes [adj, max_scale) so the decomposition can index them by scale
+      uint8_t* Ms = M + (size_t)scale_adjust * s2;
+      float* Es = E + (size_t)scale_adjust * s2;
+      source_extract(tslice, ns, g.sub, g.sub, prm.tolerance, prm.neg_comp, Ms, Es, nullptr, ext_work, nullptr, st_);
+      iuwt_recompose(Es, ns, g.sub, g.sub, scale_adjust, nullptr, rec, iuwt_work, nullptr, nullptr, st_);
+
+      // ---- CG minor loop
+      fill_f32(x, s2, 0.f, st_);
+      copy_plane(rec, r, s2, nullptr, st_);
+      copy_plane(rec, p, s2, nullptr, st_);
+      dot_partials(r, nullptr, s2, partial, nullptr, st_);
+      sum_partials(partial, gr, 1, d_rr, nullptr, st_);
+      plane_sumsq_partials(Es, 1, (size_t)ns * s2, partial, nullptr, st_);
+      sum_partials(partial, reduce_grid((size_t)ns * s2), 1, d_e2, nullptr, st_);
+      double e2 = 0.0;
+      PMS_HIP(hipMemcpyAsync(&e2, d_e2, sizeof(double), hipMemcpyDeviceToHost, st_));
+
+      int niter = 0;
+      double snr_last = 0.0, snr_current = 0.0;
+      int exit_code = EXIT_CAP;
+      while (niter < prm.minor_loop_miter) {
+        apply_op_masked(p, ap, max_scale, scale_adjust);
+        dot_partials(p, ap, s2, partial, nullptr, st_);
+        sum_partials(partial, gr, 1, d_pap, nullptr, st_);
+        hipLaunchKernelGGL(k_div, dim3(1), dim3(1), 0, st_, (const double*)d_rr, (const double*)d_pap, d_alpha);
+        PMS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_cg_xn, dim3(gr), dim3(kT), 0, st_, (const float*)x, (const float*)p,
+                           (const double*)d_alpha, xn, s2, pmax);
+        PMS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_cg_clipflag, dim3(1), dim3(1), 0, st_, (const float*)pmax, gr, prm.enforce_positivity,
+                           clip, xmin);
+        PMS_LAUNCH_CHECK();
+        if (prm.enforce_positivity) {
+          hipLaunchKernelGGL(k_cg_clip, dim3(gr), dim3(kT), 0, st_, xn, (const float*)x, p, (const double*)d_alpha,
+                             s2, (const int*)clip);
+          PMS_LAUNCH_CHECK();
+          apply_op_masked(p, ap, max_scale, scale_adjust, clip);
+        }
+        hipLaunchKernelGGL(k_cg_rn, dim3(gr), dim3(kT), 0, st_, (const float*)r, (const float*)ap,
+                           (const double*)d_alpha, rn, s2, partial);
+        PMS_LAUNCH_CHECK();
+        sum_partials(partial, gr, 1, d_rnrn, nullptr, st_);
+        hipLaunchKernelGGL(k_div, dim3(1), dim3(1), 0, st_, (const double*)d_rnrn, (const double*)d_rr, d_beta);
+        PMS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_cg_p, dim3(gr), dim3(kT), 0, st_, (const float*)rn, p, (const double*)d_beta, s2);
+        PMS_LAUNCH_CHECK();
+        // model sources and the SNR sums (main.py:413-424)
+        fft_conv(xn, g.sub, g.p_sub, spec_sub, cbuf, fft_work, nullptr, st_);
+        const long np = iuwt_decompose(cbuf, g.sub, g.sub, max_scale, scale_adjust, nullptr, nullptr, iuwt_work,
+                                       Ms, Es, partial, nullptr, st_);
+        sum_partials(partial, np, 1, d_d2, nullptr, st_);
+        double d2 = 0.0;
+        PMS_HIP(hipMemcpyAsync(&d2, d_d2, sizeof(double), hipMemcpyDeviceToHost, st_));
+        PMS_HIP(hipStreamSynchronize(st_));
+        snr_last = snr_current;
+        snr_current = 20.0 * std::log10(std::sqrt(e2) / std::sqrt(d2));
+        ++niter;
+        if (niter == 1 && snr_current > 40) {
+          ++min_scale;
+          exit_code = EXIT_FIRST_HIGH;
+          break;
+        }
+        if (snr_current > 40) {
+          std::swap(x, xn);
+          min_scale = 0;
+          exit_code = EXIT_ACCEPT;
+          break;
+        }
+        if (niter > 2 && snr_current <= snr_last) {
+          if (snr_current > 10.5) {
+            min_scale = 0;
+            exit_code = EXIT_FELL_OK;
+          } else {
+            ++min_scale;
+            exit_code = EXIT_FELL_LOW;
+          }
+          break;
+        }
+        std::swap(r, rn);
+        std::swap(x, xn);
+        std::swap(d_rr, d_rnrn);
+      }
+      trace.push_back(ev(EV_MINOR, major, niter, exit_code, 0, snr_current, 0.0));
+      if (niter == prm.minor_loop_miter && snr_current > 10.5) {
+        min_scale = 0;
+        break;
+      }
+      if (min_scale == 0) break;
+      PMS_REQUIRE(!(exit_code == EXIT_CAP && min_scale > 0),
+                  "minor loop hit its cap at SNR <= 10.5 with min_scale > 0: the reference loops forever here");
+    }
+
+    if (min_scale == S) {
+      trace.push_back(ev(EV_POOR, major, 0, 0, 0, 0.0, 0.0));
+      break;
+    }
+    if (max_coeff > 0) {
+      hipLaunchKernelGGL(k_model_add, dim3(grid_for(s2)), dim3(kT), 0, st_, model_local, g.n, g.lo, g.lo, g.sub,
+                         (const float*)x, prm.loop_gain);
+      PMS_LAUNCH_CHECK();
+      const double std_last = std_current;
+      std_current = residual_and_std(dirty_dev, model_local);
+      std_ratio = (std_last - std_current) / std_last;
+      const bool reverted = std_ratio < 0;
+      if (reverted) {
+        hipLaunchKernelGGL(k_model_add, dim3(grid_for(s2)), dim3(kT), 0, st_, model_local, g.n, g.lo, g.lo, g.sub,
+                           (const float*)x, -prm.loop_gain);
+        PMS_LAUNCH_CHECK();
+        residual_and_std(dirty_dev, model_local);
+      }
+      trace.push_back(ev(EV_UPDATE, major, reverted ? 1 : 0, 0, 0, std_current, std_ratio));
+      // dirty_sub already holds residual[sub] (written by residual_and_std)
+      plane_max(dirty_sub, 1, s2, 0, pmax, nullptr, st_);
+      max_final(pmax, pmx, 1, (float*)(scal + 60), nullptr, nullptr, nullptr, st_);
+      PMS_HIP(hipMemcpyAsync(&dmax, scal + 60, sizeof(float), hipMemcpyDeviceToHost, st_));
+      PMS_HIP(hipStreamSynchronize(st_));
+      ++major;
+    }
+    if (major == 0) {
+      trace.push_back(ev(EV_NOWORK, 0, 0, 0, 0, 0.0, 0.0));
+      *complete = 1;
+      break;
+    }
+  }
+  if (major > 0) {
+    hipLaunchKernelGGL(k_add_inplace, dim3(grid_for(N2)), dim3(kT), 0, st_, model_dev, (const float*)model_local, N2);
+    PMS_LAUNCH_CHECK();
+    copy_plane(resid, residual_dev, N2, nullptr, st_);
+  }
+  PMS_HIP(hipStreamSynchronize(st_));
+  return major;
+}
+
+// Ap <- R(M . W(H v)) over scales [adj, max_scale); masks are indexed by scale
+void Job::apply_op_masked(const float* v, float* out, int max_scale, int adj, const int* pred) {
+  const Geometry& g = geo_;
+  fft_conv(v, g.sub, g.p_sub, spec_sub, cbuf, fft_work, pred, st_);
+  iuwt_decompose(cbuf, g.sub, g.sub, max_scale, adj, Dcg, nullptr, iuwt_work, M + (size_t)adj * g.sub * g.sub,
+                 nullptr, nullptr, pred, st_);
+  iuwt_recompose(Dcg, max_scale - adj, g.sub, g.sub, adj, nullptr, out, iuwt_work + 0, nullptr, pred, st_);
+}
+
+// main.py:523-599
+void Job::moresane_by_scale(const MoresaneParams& prm, int start_scale, int stop_scale, const float* dirty_dev,
+                            const float* psf_dev, const float* mask_dev, float* model_dev, float* residual_dev,
+                            Trace& trace) {
+  const int n = prm.n;
+  const size_t N2 = (size_t)n * n;
+  // the dirty image of each call is the previous residual; keep a private copy so the caller's dirty stays intact
+  float* cur_dirty = (float*)by_scale_arena_.alloc(N2 * sizeof(float));
+  copy_plane(dirty_dev, cur_dirty, N2, nullptr, st_);
+  int complete = 0;
+  int scale_count = start_scale;
+  while (!complete) {
+    trace.push_back(ev(EV_BYSCALE, 0, scale_count, 0, 0, 0.0, 0.0));
+    moresane(prm, scale_count, cur_dirty, psf_dev, mask_dev, model_dev, residual_dev, &complete, trace);
+    copy_plane(residual_dev, cur_dirty, N2, nullptr, st_);
+    ++scale_count;
+    if (scale_count > std::log2((double)n) - 1) break;
+    if (scale_count > stop_scale) break;
+  }
+  PMS_HIP(hipStreamSynchronize(st_));
+  by_scale_arena_.release();
+}
+
+}  // namespace pms

@@ -1,0 +1,462 @@
+// Isotropic undecimated wavelet transform (B3-spline a trous) for gfx950.
+//
+// Semantics follow the reference serial IUWT (pymoresane/iuwt.py:66-191):
+//   smoothing h_s  = separable [1,4,6,4,1]/16 with taps at 0, +-2^s, +-2^(s+1),
+//                    half-sample symmetric extension at the borders;
+//   decomposition  C = h_s*C0, w_s = C0 - h_s*C, C0 <- C   (iuwt.py:102-106)
+//   recomposition  R <- h_s*R + w_s from coarse to fine    (iuwt.py:142-147)
+//
+// Kernel design.  A dilated filter only couples pixels whose row (and
+// column) indices agree modulo d = 2^s, so a workgroup owns one row phase p
+// (rows p, p+d, p+2d, ...) and a "comb" of columns: Q consecutive column
+// phases (a contiguous Q-float chunk) repeated every d columns for TJ teeth.
+// In that polyphase frame every scale is an undilated 5-tap filter with a
+// halo of a few teeth, whatever d is.  The workgroup walks down its row
+// phase: each input row of the comb (plus halo teeth) is staged in LDS, the
+// horizontal taps are read from LDS, and the vertical taps come from
+// per-thread register windows that slide one row per step.  Input rows are
+// prefetched PF rows ahead into registers so several rows are in flight per
+// workgroup.
+//
+// One decomposition scale is one launch: from C0 it produces C = H C0 (5-tap
+// in both axes) and w = C0 - H(H C0), using the 9-tap filter B3*B3 for the
+// doubled smoothing (exact: on the symmetric extension, which is periodic
+// with period 2N, the two passes compose into one circular filter).  The
+// detail epilogue optionally multiplies by a {0,1} byte mask and/or reduces
+// sum((E - w)^2) against a reference plane instead of storing w.
+#include "pms_internal.h"
+
+namespace pms {
+namespace {
+
+constexpr int kThreads = 256;
+
+__constant__ float c_h5[5] = {1.f / 16, 4.f / 16, 6.f / 16, 4.f / 16, 1.f / 16};
+__constant__ float c_h9[9] = {1.f / 256, 8.f / 256, 28.f / 256, 56.f / 256, 70.f / 256,
+                              56.f / 256, 28.f / 256, 8.f / 256, 1.f / 256};
+
+struct ScaleParams {
+  const float* in;        // C0 (decomp/smooth) or R (recomp)
+  float* out_c;           // C (decomp/smooth) or R' (recomp); may be null in decomp
+  float* out_w;           // decomp: detail output (null = not stored)
+  const float* w_in;      // recomp: detail plane added after smoothing
+  const uint8_t* mask;    // decomp: mask applied to w; recomp: mask applied to w_in (null = none)
+  const float* ref;       // decomp: reduce sum((ref - w)^2) into partial[] (null = off)
+  double* partial;        // one double per workgroup when ref != null
+  const int* pred;        // skip the launch body when *pred == 0 (null = always run)
+  int h, w;               // plane shape (row-major, contiguous)
+  int d;                  // dilation 2^s
+  int q_log2, tj;         // comb: Q = 1<<q_log2 phases x TJ teeth (Q*TJ <= 256)
+  int k_rows;             // output rows (in phase units) per workgroup walk
+  int n_cg, n_tb, n_seg;  // grid decomposition
+};
+
+template <int MODE>
+struct ModeTraits;
+template <>
+struct ModeTraits<IUWT_DECOMP> {
+  static constexpr int H = 4;  // 9-tap halo
+};
+template <>
+struct ModeTraits<IUWT_SMOOTH> {
+  static constexpr int H = 2;
+};
+template <>
+struct ModeTraits<IUWT_RECOMP> {
+  static constexpr int H = 2;
+};
+
+template <int MODE, int PF, int ML>
+__global__ __launch_bounds__(kThreads) void k_iuwt_scale(ScaleParams a) {
+  constexpr int kMaxLoads = ML;  // LDS-staging loads per thread per comb row
+  constexpr int H = ModeTraits<MODE>::H;
+  constexpr int WA = (MODE == IUWT_DECOMP) ? 7 : 5;  // window of horizontal 5-tap results
+  if (a.pred && *a.pred == 0) return;
+
+  extern __shared__ float lds[];
+  const int Q = 1 << a.q_log2;
+  const int TJ = a.tj;
+  const int row_elems = (TJ + 2 * H) * Q;
+  float* buf0 = lds;
+  float* buf1 = lds + row_elems;
+
+  // decode the (remapped) block id: consecutive logical ids walk consecutive segments of one strip
+  const unsigned nblk = gridDim.x;
+  unsigned b = xcd_remap(blockIdx.x, nblk);
+  const int seg = b % a.n_seg;
+  b /= a.n_seg;
+  const int tb = b % a.n_tb;
+  b /= a.n_tb;
+  const int cg = b % a.n_cg;
+  const int p = b / a.n_cg;  // row phase
+  const int d = a.d;
+
+  const int t = threadIdx.x;
+  const int q = t & (Q - 1);
+  const int j = t >> a.q_log2;
+  const bool thread_live = j < TJ;
+  const int col = cg * Q + q + (tb * TJ + j) * d;
+  const bool col_ok = thread_live && col < a.w;
+
+  // rows of this phase: r = p + i*d for i in [0, mr)
+  const int mr = (a.h - p + d - 1) / d;
+  const int o_begin = seg * a.k_rows;
+  const int o_end = min(o_begin + a.k_rows, mr);
+  const int i_begin = o_begin - H;
+  const int n_steps = (o_end - o_begin) + 2 * H;
+  const int w = a.w;
+
+  // per-thread staging slots for one comb row
+  int ld_col[kMaxLoads];
+#pragma unroll
+  for (int m = 0; m < kMaxLoads; ++m) {
+    const int e = t + m * kThreads;
+    ld_col[m] = -1;
+    if (e < row_elems) {
+      const int jj = e >> a.q_log2;
+      const int qq = e & (Q - 1);
+      ld_col[m] = reflect_idx(cg * Q + qq + (tb * TJ + jj - H) * d, w);
+    }
+  }
+
+  float regs[PF][kMaxLoads];
+  auto load_row = [&](float (&dst)[kMaxLoads], int i) {
+    const int r = reflect_idx(p + i * d, a.h);
+    const float* src = a.in + (size_t)r * w;
+#pragma unroll
+    for (int m = 0; m < kMaxLoads; ++m)
+      if (ld_col[m] >= 0) dst[m] = src[ld_col[m]];
+  };
+  auto store_row = [&](const float (&src)[kMaxLoads], float* dstbuf) {
+#pragma unroll
+    for (int m = 0; m < kMaxLoads; ++m)
+      if (ld_col[m] >= 0) dstbuf[t + m * kThreads] = src[m];
+  };
+
+  float wa[WA];
+  float wb[9];
+  float wx[5];
+#pragma unroll
+  for (int k = 0; k < WA; ++k) wa[k] = 0.f;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) wb[k] = 0.f;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) wx[k] = 0.f;
+  double acc = 0.0;
+
+  // prologue: PF rows in flight, first row staged
+#pragma unroll
+  for (int u = 0; u < PF; ++u)
+    if (u < n_steps) load_row(regs[u], i_begin + u);
+  store_row(regs[0], buf0);
+  __syncthreads();
+
+  for (int s0 = 0; s0 < n_steps; s0 += PF) {
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      const int s = s0 + u;
+      if (s < n_steps) {
+        const int i = i_begin + s;
+        float* cur = (s & 1) ? buf1 : buf0;
+        float* nxt = (s & 1) ? buf0 : buf1;
+        if (s + PF < n_steps) load_row(regs[u], i + PF);
+
+        // horizontal taps from LDS
+        const float* rowp = cur + (j + H) * Q + q;
+        float ha = 0.f, hb = 0.f, hx = 0.f;
+        if (thread_live) {
+#pragma unroll
+          for (int k = -2; k <= 2; ++k) ha = fmaf(c_h5[k + 2], rowp[k * Q], ha);
+          if (MODE == IUWT_DECOMP) {
+#pragma unroll
+            for (int k = -4; k <= 4; ++k) hb = fmaf(c_h9[k + 4], rowp[k * Q], hb);
+            hx = rowp[0];
+          }
+        }
+        // slide the vertical windows
+#pragma unroll
+        for (int k = 0; k < WA - 1; ++k) wa[k] = wa[k + 1];
+        wa[WA - 1] = ha;
+        if (MODE == IUWT_DECOMP) {
+#pragma unroll
+          for (int k = 0; k < 8; ++k) wb[k] = wb[k + 1];
+          wb[8] = hb;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) wx[k] = wx[k + 1];
+          wx[4] = hx;
+        }
+
+        if (s >= 2 * H && col_ok) {
+          const int o = i - H;                      // output row in phase units
+          const size_t off = (size_t)(p + o * d) * w + col;
+          float c = 0.f;
+#pragma unroll
+          for (int k = 0; k < 5; ++k) c = fmaf(c_h5[k], wa[k], c);
+          if (MODE == IUWT_DECOMP) {
+            float c1 = 0.f;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) c1 = fmaf(c_h9[k], wb[k], c1);
+            float wv = wx[0] - c1;
+            if (a.mask && a.mask[off] == 0) wv = 0.f;
+            if (a.out_c) a.out_c[off] = c;
+            if (a.out_w) a.out_w[off] = wv;
+            if (a.ref) {
+              const float df = a.ref[off] - wv;
+              acc = fma((double)df, (double)df, acc);
+            }
+          } else if (MODE == IUWT_SMOOTH) {
+            a.out_c[off] = c;
+          } else {
+            float wi = a.w_in[off];
+            if (a.mask && a.mask[off] == 0) wi = 0.f;
+            a.out_c[off] = c + wi;
+          }
+        }
+        if (s + 1 < n_steps) store_row(regs[(u + 1) % PF], nxt);
+        __syncthreads();
+      }
+    }
+  }
+
+  if (MODE == IUWT_DECOMP && a.ref) {
+    const double tot = block_sum_f64(acc);
+    if (t == 0) a.partial[blockIdx.x] = tot;
+  }
+}
+
+struct Plan {
+  int q_log2, tj, k_rows, n_cg, n_tb, n_seg, blocks, lds_bytes;
+};
+
+int env_int(const char* name, int dflt) {
+  const char* v = getenv(name);
+  return v ? atoi(v) : dflt;
+}
+
+Plan make_plan(int h, int w, int scale, int halo) {
+  Plan pl;
+  const int d = 1 << scale;
+  const int teeth = ceil_div(w, d);
+  const int qmax = env_int("PMS_IUWT_QMAX", 16);
+  int q = d < qmax ? d : qmax;
+  int tj = kThreads / q;
+  int tp = 1;
+  while (tp < teeth) tp <<= 1;
+  if (tj > tp) tj = tp;
+  while (q * tj < kThreads && q < d) q <<= 1;  // widen the chunk when teeth are few
+  pl.q_log2 = ilog2(q);
+  pl.tj = tj;
+  const int mr = ceil_div(h, d);
+  // aim for enough workgroups to fill 256 CUs several times over
+  const int strips = d * ceil_div(d, q) * ceil_div(teeth, tj);
+  int k = env_int("PMS_IUWT_KROWS", 0);
+  if (k <= 0) {
+    const long target = 2048;
+    long segs = (target + strips - 1) / strips;
+    if (segs < 1) segs = 1;
+    k = ceil_div(mr, (int)segs);
+    const int kmin = 4 * halo;
+    if (k < kmin) k = kmin;
+  }
+  if (k > mr) k = mr;
+  pl.k_rows = k;
+  pl.n_cg = ceil_div(d, q);
+  pl.n_tb = ceil_div(teeth, tj);
+  pl.n_seg = ceil_div(mr, k);
+  pl.blocks = d * pl.n_cg * pl.n_tb * pl.n_seg;
+  pl.lds_bytes = 2 * (tj + 2 * halo) * q * (int)sizeof(float);
+  return pl;
+}
+
+template <int MODE, int ML>
+void launch_pf(ScaleParams& sp, const Plan& pl, hipStream_t st) {
+  const int pf = env_int("PMS_IUWT_PF", 3);
+  dim3 grid(pl.blocks), block(kThreads);
+  if (pf <= 1)
+    hipLaunchKernelGGL((k_iuwt_scale<MODE, 1, ML>), grid, block, pl.lds_bytes, st, sp);
+  else if (pf == 2)
+    hipLaunchKernelGGL((k_iuwt_scale<MODE, 2, ML>), grid, block, pl.lds_bytes, st, sp);
+  else
+    hipLaunchKernelGGL((k_iuwt_scale<MODE, 3, ML>), grid, block, pl.lds_bytes, st, sp);
+  PMS_LAUNCH_CHECK();
+}
+
+template <int MODE>
+void launch_mode(ScaleParams& sp, const Plan& pl, int halo, hipStream_t st) {
+  const int elems = (pl.tj + 2 * halo) << pl.q_log2;
+  if (elems <= 2 * kThreads)
+    launch_pf<MODE, 2>(sp, pl, st);
+  else if (elems <= 4 * kThreads)
+    launch_pf<MODE, 4>(sp, pl, st);
+  else
+    launch_pf<MODE, 8>(sp, pl, st);
+}
+
+}  // namespace
+
+void check_iuwt_scale(int h, int w, int scale) {
+  PMS_REQUIRE(h >= 2 && w >= 2, "IUWT planes must be at least 2x2");
+  PMS_REQUIRE(scale >= 0 && scale < 30, "IUWT scale out of range");
+  PMS_REQUIRE((2L << scale) <= h && (2L << scale) <= w,
+              "IUWT scale " + std::to_string(scale) + " needs 2^(scale+1) <= image side (" +
+                  std::to_string(h) + "x" + std::to_string(w) + ")");
+}
+
+int iuwt_scale(const IuwtScaleCall& c, hipStream_t st) {
+  check_iuwt_scale(c.h, c.w, c.scale);
+  const int halo = (c.mode == IUWT_DECOMP) ? 4 : 2;
+  Plan pl = make_plan(c.h, c.w, c.scale, halo);
+  PMS_REQUIRE((pl.tj + 2 * halo) * (1 << pl.q_log2) <= 8 * kThreads, "IUWT tile too wide");
+  ScaleParams sp;
+  sp.in = c.in;
+  sp.out_c = c.out_c;
+  sp.out_w = c.out_w;
+  sp.w_in = c.w_in;
+  sp.mask = c.mask;
+  sp.ref = c.ref;
+  sp.partial = c.partial;
+  sp.pred = c.pred;
+  sp.h = c.h;
+  sp.w = c.w;
+  sp.d = 1 << c.scale;
+  sp.q_log2 = pl.q_log2;
+  sp.tj = pl.tj;
+  sp.k_rows = pl.k_rows;
+  sp.n_cg = pl.n_cg;
+  sp.n_tb = pl.n_tb;
+  sp.n_seg = pl.n_seg;
+  if (c.mode == IUWT_DECOMP)
+    launch_mode<IUWT_DECOMP>(sp, pl, halo, st);
+  else if (c.mode == IUWT_SMOOTH)
+    launch_mode<IUWT_SMOOTH>(sp, pl, halo, st);
+  else
+    launch_mode<IUWT_RECOMP>(sp, pl, halo, st);
+  return pl.blocks;
+}
+
+int iuwt_scale_blocks(int h, int w, int scale, int mode) {
+  return make_plan(h, w, scale, mode == IUWT_DECOMP ? 4 : 2).blocks;
+}
+
+// Full decomposition.  detail: (scale_count - scale_adjust) planes; smoothed
+// may be null; work: two planes of scratch.  mask/ref (optional) are
+// per-detail-plane arrays laid out like detail.  Returns the number of
+// partial sums written (ref != null), in plane order.
+long iuwt_decompose(const float* in, int h, int w, int scale_count, int scale_adjust, float* detail,
+                    float* smoothed, float* work, const uint8_t* mask, const float* ref,
+                    double* partial, const int* pred, hipStream_t st) {
+  PMS_REQUIRE(scale_adjust >= 0 && scale_count > scale_adjust, "need scale_count > scale_adjust >= 0");
+  check_iuwt_scale(h, w, scale_count - 1);
+  const size_t plane = (size_t)h * w;
+  float* ping[2] = {work, work + plane};
+  const float* cur = in;
+  int slot = 0;
+  for (int s = 0; s < scale_adjust; ++s) {
+    IuwtScaleCall c{};
+    c.mode = IUWT_SMOOTH;
+    c.in = cur;
+    c.out_c = ping[slot];
+    c.h = h;
+    c.w = w;
+    c.scale = s;
+    c.pred = pred;
+    iuwt_scale(c, st);
+    cur = ping[slot];
+    slot ^= 1;
+  }
+  long np = 0;
+  for (int s = scale_adjust; s < scale_count; ++s) {
+    const int k = s - scale_adjust;
+    const bool last = (s == scale_count - 1);
+    IuwtScaleCall c{};
+    c.mode = IUWT_DECOMP;
+    c.in = cur;
+    c.out_c = last ? smoothed : ping[slot];
+    c.out_w = detail ? detail + (size_t)k * plane : nullptr;
+    c.mask = mask ? mask + (size_t)k * plane : nullptr;
+    c.ref = ref ? ref + (size_t)k * plane : nullptr;
+    c.partial = ref ? partial + np : nullptr;
+    c.h = h;
+    c.w = w;
+    c.scale = s;
+    c.pred = pred;
+    const int nb = iuwt_scale(c, st);
+    if (ref) np += nb;
+    if (!last) {
+      cur = ping[slot];
+      slot ^= 1;
+    }
+  }
+  return np;
+}
+
+long iuwt_decompose_partials(int h, int w, int scale_count, int scale_adjust) {
+  long np = 0;
+  for (int s = scale_adjust; s < scale_count; ++s) np += iuwt_scale_blocks(h, w, s, IUWT_DECOMP);
+  return np;
+}
+
+// Full recomposition of nplanes detail planes (scales scale_adjust ..
+// scale_adjust+nplanes-1) with optional smoothed base and optional mask on the
+// detail planes.  work: two planes of scratch; out: one plane.
+void iuwt_recompose(const float* detail, int nplanes, int h, int w, int scale_adjust,
+                    const float* smoothed, float* out, float* work, const uint8_t* mask,
+                    const int* pred, hipStream_t st) {
+  PMS_REQUIRE(nplanes >= 1 && scale_adjust >= 0, "recomposition needs at least one detail plane");
+  const int top = nplanes + scale_adjust;
+  check_iuwt_scale(h, w, top - 1);
+  const size_t plane = (size_t)h * w;
+  float* ping[2] = {work, work + plane};
+  int slot = 0;
+  const float* cur = smoothed;
+  int s = top - 1;
+  if (!smoothed) {
+    // R starts at zero, so the coarsest step is the detail plane itself
+    if (!mask) {
+      cur = detail + (size_t)(s - scale_adjust) * plane;
+    } else {
+      masked_copy(detail + (size_t)(s - scale_adjust) * plane, mask + (size_t)(s - scale_adjust) * plane,
+                  ping[slot], plane, pred, st);
+      cur = ping[slot];
+      slot ^= 1;
+    }
+    --s;
+  }
+  int remaining = (s - scale_adjust + 1) + scale_adjust;  // launches still to issue
+  if (remaining == 0) {
+    copy_plane(cur, out, plane, pred, st);
+    return;
+  }
+  for (; s >= scale_adjust; --s) {
+    --remaining;
+    IuwtScaleCall c{};
+    c.mode = IUWT_RECOMP;
+    c.in = cur;
+    c.out_c = remaining == 0 ? out : ping[slot];
+    c.w_in = detail + (size_t)(s - scale_adjust) * plane;
+    c.mask = mask ? mask + (size_t)(s - scale_adjust) * plane : nullptr;
+    c.h = h;
+    c.w = w;
+    c.scale = s;
+    c.pred = pred;
+    iuwt_scale(c, st);
+    cur = c.out_c;
+    slot ^= 1;
+  }
+  for (s = scale_adjust - 1; s >= 0; --s) {
+    --remaining;
+    IuwtScaleCall c{};
+    c.mode = IUWT_SMOOTH;
+    c.in = cur;
+    c.out_c = remaining == 0 ? out : ping[slot];
+    c.h = h;
+    c.w = w;
+    c.scale = s;
+    c.pred = pred;
+    iuwt_scale(c, st);
+    cur = c.out_c;
+    slot ^= 1;
+  }
+}
+
+}  // namespace pms

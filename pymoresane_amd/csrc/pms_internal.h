@@ -1,0 +1,162 @@
+// Internal (C++) interfaces between the kernel files of the MORESANE HIP library.
+#pragma once
+
+#include "pms_common.h"
+
+namespace pms {
+
+// ----------------------------------------------------------------- reductions
+// Deterministic block-wide sums/max (fixed shuffle tree, then a fixed LDS order).
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ float wave_max_f32(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+__device__ __forceinline__ float wave_min_f32(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Block sum of a double; valid in thread 0.  Block size must be a multiple of 64, <= 1024.
+__device__ __forceinline__ double block_sum_f64(double v) {
+  __shared__ double red_f64[16];
+  v = wave_sum_f64(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red_f64[wid] = v;
+  __syncthreads();
+  double tot = 0.0;
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x >> 6;
+    for (int k = 0; k < nw; ++k) tot += red_f64[k];
+  }
+  return tot;
+}
+
+__device__ __forceinline__ float block_max_f32(float v) {
+  __shared__ float red_f32[16];
+  v = wave_max_f32(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red_f32[wid] = v;
+  __syncthreads();
+  float m = -INFINITY;
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x >> 6;
+    for (int k = 0; k < nw; ++k) m = fmaxf(m, red_f32[k]);
+  }
+  return m;
+}
+
+__device__ __forceinline__ float block_min_f32(float v) {
+  __shared__ float red_f32n[16];
+  v = wave_min_f32(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red_f32n[wid] = v;
+  __syncthreads();
+  float m = INFINITY;
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x >> 6;
+    for (int k = 0; k < nw; ++k) m = fminf(m, red_f32n[k]);
+  }
+  return m;
+}
+
+// ----------------------------------------------------------------------- IUWT
+enum IuwtMode { IUWT_DECOMP = 0, IUWT_SMOOTH = 1, IUWT_RECOMP = 2 };
+
+struct IuwtScaleCall {
+  int mode;
+  const float* in;
+  float* out_c;
+  float* out_w;
+  const float* w_in;
+  const uint8_t* mask;
+  const float* ref;
+  double* partial;
+  const int* pred;
+  int h, w, scale;
+};
+
+void check_iuwt_scale(int h, int w, int scale);
+int iuwt_scale(const IuwtScaleCall& c, hipStream_t st);  // returns launched blocks
+int iuwt_scale_blocks(int h, int w, int scale, int mode);
+long iuwt_decompose(const float* in, int h, int w, int scale_count, int scale_adjust, float* detail,
+                    float* smoothed, float* work, const uint8_t* mask, const float* ref,
+                    double* partial, const int* pred, hipStream_t st);
+long iuwt_decompose_partials(int h, int w, int scale_count, int scale_adjust);
+void iuwt_recompose(const float* detail, int nplanes, int h, int w, int scale_adjust,
+                    const float* smoothed, float* out, float* work, const uint8_t* mask,
+                    const int* pred, hipStream_t st);
+
+// ------------------------------------------------------------------------ FFT
+// Device twiddle table exp(-2 pi i k / kFftMaxLen), k in [0, kFftMaxLen).
+constexpr int kFftMaxLog2 = 14;
+constexpr int kFftMaxLen = 1 << kFftMaxLog2;
+void fft_init_device(int device);
+const float2* fft_twiddles();
+
+// Spectrum layout: (P/2+1) frequency columns, each holding P values along the
+// other axis in the library's digit-reversed FFT order, already carrying the
+// P/2 spatial roll that centres the PSF at the origin.
+size_t fft_spectrum_elems(int p);
+size_t fft_conv_work_elems(int n, int p);  // float2 elements of scratch for fft_conv
+// psf (q x q) embedded centred in a p x p frame, rolled by -p/2 -> spectrum.
+void fft_psf_spectrum(const float* psf, int q, int p, float2* spec, float2* work, hipStream_t st);
+// out (n x n) = image (n x n) convolved with the PSF of spectrum spec (p = n or 2n).
+void fft_conv(const float* in, int n, int p, const float2* spec, float* out, float2* work,
+              const int* pred, hipStream_t st);
+// spectrum -> digit-reversed column-major layout (host arrays, numpy rfft2 convention)
+void fft_spectrum_from_numpy_host(const float* re_im, int p, float2* spec_host);
+int fft_perm(int k, int p);  // storage position of frequency k in a length-p column
+
+// ------------------------------------------------------------------- toolbox
+void copy_plane(const float* src, float* dst, size_t n, const int* pred, hipStream_t st);
+void masked_copy(const float* src, const uint8_t* mask, float* dst, size_t n, const int* pred,
+                 hipStream_t st);
+void fill_f32(float* p, size_t n, float v, hipStream_t st);
+void fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t st);
+// dst (n x n) = src[r0.., c0..] (row stride ld), times mul (same layout as src) when non-null
+void window_copy(const float* src, int ld, int r0, int c0, int n, const float* mul, float* dst, hipStream_t st);
+int reduce_grid(size_t n);      // number of partials written by dot/sumsq reductions over n elements
+int threshold_grid(size_t plane);  // partial maxima per plane
+
+// Pixels entering the MAD estimate: inside the include rectangle and outside
+// the exclude rectangle (iuwt_toolbox.py:31-43, Python slice semantics).
+struct NoiseMask {
+  bool masked;
+  int inc_r0, inc_r1, inc_c0, inc_c1;
+  int exc_r0, exc_r1, exc_c0, exc_c1;
+  long count;
+};
+NoiseMask noise_mask(int h, int w, int edge_excl, int int_excl);
+size_t mad_work_bytes(int S);
+void mad_work_init(void* work, int S, hipStream_t st);  // once per workspace (histograms start at 0)
+void mad_thresholds(const float* planes, int S, int h, int w, const NoiseMask& nm, double* thr, void* work,
+                    const int* pred, hipStream_t st);
+// out = in where |in| > sigma*thr[s] else 0, zeroing a per-scale border (border may be null);
+// pmax receives threshold_grid(plane) partial maxima per plane.
+void apply_threshold(const float* in, float* out, int S, int h, int w, const double* thr, double sigma,
+                     const int* border, float* pmax, const int* pred, hipStream_t st);
+void plane_max(const float* in, int S, size_t plane, int use_abs, float* pmax, const int* pred, hipStream_t st);
+void max_final(const float* pmax, int nb, int S, float* out_max, const double* energy, double* nsm, const int* pred,
+               hipStream_t st);
+size_t extract_work_bytes(int S, int h, int w);
+void extract_work_init(void* work, int S, int h, int w, hipStream_t st);
+void source_extract(const float* w, int S, int h, int wd, double tol, int neg_comp, uint8_t* mask, float* extracted,
+                    int* labels_out, void* work, const int* pred, hipStream_t st);
+void dot_partials(const float* a, const float* b, size_t n, double* partial, const int* pred, hipStream_t st);
+void sum_partials(const double* partial, long np, int groups, double* out, const int* pred, hipStream_t st);
+void plane_sumsq_partials(const float* a, int S, size_t plane, double* partial, const int* pred, hipStream_t st);
+void mean_std(const float* a, size_t n, double* partial, double* sum_out, double* dev2_out, hipStream_t st);
+
+}  // namespace pms

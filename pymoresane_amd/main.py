@@ -1,0 +1,162 @@
+"""DataImage / FitsImage with the MORESANE deconvolution running on the MI355X.
+
+Drop-in for pymoresane/main.py:18-616: ``DataImage(dirty, psf, mask, cdelt1,
+cdelt2)`` and ``FitsImage(dirty.fits, psf.fits, mask=None)`` with
+``moresane(...)`` and ``moresane_by_scale(...)`` taking the reference's
+keyword arguments (main.py:53-57, 523-527).  The HIP engine in libpmship.so
+runs the whole major/minor loop with every array resident in HBM; this
+module uploads the inputs, calls it, and reads back model, residual and the
+decision trace.  ``decom_mode``, ``conv_device`` and ``extraction_mode`` must
+be ``'hip'`` (``'gpu'`` is an alias; ``all_on_gpu=True`` selects it too);
+``core_count`` is accepted and ignored.
+"""
+
+import ctypes
+import logging
+
+import numpy as np
+
+from . import _lib
+from ._lib import DeviceArray, MoresaneParams, TraceEvent, check
+from .iuwt import check_mode
+
+logger = logging.getLogger(__name__)
+
+TRACE_CAP = 1 << 16
+
+
+def decode_trace(events, n):
+    """Decision trace as tuples in the oracle's format."""
+    out = []
+    for e in events[:n]:
+        k = e.kind
+        if k == _lib.EV_SCALES:
+            out.append(("scales", e.major, e.a, e.b, e.c, e.x))
+        elif k == _lib.EV_MINOR:
+            out.append(("minor", e.major, e.a, e.b, e.x))
+        elif k == _lib.EV_UPDATE:
+            out.append(("update", e.major, e.x, e.y, bool(e.a)))
+        elif k == _lib.EV_NOCOEFF:
+            out.append(("nocoeff", e.major))
+        elif k == _lib.EV_POOR:
+            out.append(("poor", e.major))
+        elif k == _lib.EV_NOWORK:
+            out.append(("nowork",))
+        elif k == _lib.EV_BYSCALE:
+            out.append(("byscale", e.a))
+    return out
+
+
+def make_params(n, psf_n, subregion, scale_count, sigma_level, loop_gain, tolerance, accuracy, major_loop_miter,
+                minor_loop_miter, conv_mode, enforce_positivity, edge_suppression, edge_offset, flux_threshold,
+                neg_comp, edge_excl, int_excl):
+    if conv_mode not in ("linear", "circular"):
+        raise ValueError("conv_mode must be 'linear' or 'circular', got %r" % (conv_mode,))
+    prm = MoresaneParams()
+    prm.n, prm.psf_n = int(n), int(psf_n)
+    prm.subregion = 0 if subregion is None else int(subregion)
+    prm.scale_count = 0 if scale_count is None else int(scale_count)
+    prm.sigma_level, prm.loop_gain = float(sigma_level), float(loop_gain)
+    prm.tolerance, prm.accuracy = float(tolerance), float(accuracy)
+    prm.major_loop_miter, prm.minor_loop_miter = int(major_loop_miter), int(minor_loop_miter)
+    prm.conv_mode = 0 if conv_mode == "linear" else 1
+    prm.enforce_positivity = int(bool(enforce_positivity))
+    prm.edge_suppression = int(bool(edge_suppression))
+    prm.edge_offset = int(edge_offset)
+    prm.flux_threshold = float(flux_threshold)
+    prm.neg_comp = int(bool(neg_comp))
+    prm.edge_excl, prm.int_excl = int(edge_excl), int(int_excl)
+    return prm
+
+
+class DataImage(object):
+    """Base class managing one deconvolution (pymoresane/main.py:18-49)."""
+
+    def __init__(self, dirty_data, psf_data, mask_data, cdelt1, cdelt2):
+        self.dirty_data = np.asarray(dirty_data).astype(np.float32)
+        self.psf_data = np.asarray(psf_data).astype(np.float32)
+        self.mask_name = None
+        self.mask = None if mask_data is None else np.asarray(mask_data, dtype=np.float32)
+        self.dirty_data_shape = self.dirty_data.shape
+        self.psf_data_shape = self.psf_data.shape
+        self.complete = False
+        self.model = np.zeros_like(self.dirty_data)
+        self.residual = np.copy(self.dirty_data)
+        self.restored = np.zeros_like(self.dirty_data)
+        self.cdelt1 = cdelt1
+        self.cdelt2 = cdelt2
+        self.trace = []
+
+    # ----------------------------------------------------------------------------- helpers
+    def _check_modes(self, all_on_gpu, decom_mode, conv_device, extraction_mode):
+        if all_on_gpu:
+            decom_mode = conv_device = extraction_mode = "hip"
+        check_mode(decom_mode, "decom_mode")
+        check_mode(conv_device, "conv_device")
+        check_mode(extraction_mode, "extraction_mode")
+
+    def _check_shapes(self):
+        n = self.dirty_data_shape[0]
+        if len(self.dirty_data_shape) != 2 or self.dirty_data_shape[1] != n:
+            raise ValueError("the dirty image must be square, got %s" % (self.dirty_data_shape,))
+        if n % 2 == 1:
+            logger.error("Image size is uneven. Please use even dimensions.")
+            raise ValueError("Image size is uneven. Please use even dimensions.")
+        pn = self.psf_data_shape[0]
+        if self.psf_data_shape != (pn, pn) or pn not in (n, 2 * n):
+            raise ValueError("the PSF must be square and the dirty size or twice it, got %s" % (self.psf_data_shape,))
+
+    def _run(self, by_scale, start_scale, stop_scale, prm):
+        lib = _lib.lib()
+        dirty = DeviceArray.from_host(self.dirty_data)
+        psf = DeviceArray.from_host(self.psf_data)
+        mask = DeviceArray.from_host(self.mask) if self.mask is not None else None
+        model = DeviceArray.from_host(self.model)
+        resid = DeviceArray.from_host(self.residual)
+        trace = (TraceEvent * TRACE_CAP)()
+        tlen = ctypes.c_int(0)
+        complete = ctypes.c_int(0)
+        try:
+            if by_scale:
+                check(lib.pms_moresane_by_scale(ctypes.byref(prm), int(start_scale), int(stop_scale), dirty.ptr,
+                                                psf.ptr, mask.ptr if mask else None, model.ptr, resid.ptr, trace,
+                                                TRACE_CAP, ctypes.byref(tlen), None))
+            else:
+                check(lib.pms_moresane(ctypes.byref(prm), dirty.ptr, psf.ptr, mask.ptr if mask else None, model.ptr,
+                                       resid.ptr, ctypes.byref(complete), trace, TRACE_CAP, ctypes.byref(tlen), None))
+        finally:
+            self.trace.extend(decode_trace(trace, min(tlen.value, TRACE_CAP)))
+        self.model = model.get()
+        self.residual = resid.get()
+        if not by_scale:
+            self.complete = bool(complete.value)
+        return tlen.value
+
+    # ----------------------------------------------------------------------------- API
+    def moresane(self, subregion=None, scale_count=None, sigma_level=4, loop_gain=0.1, tolerance=0.75, accuracy=1e-6,
+                 major_loop_miter=100, minor_loop_miter=30, all_on_gpu=False, decom_mode="hip", core_count=1,
+                 conv_device="hip", conv_mode="linear", extraction_mode="hip", enforce_positivity=False,
+                 edge_suppression=False, edge_offset=0, flux_threshold=0, neg_comp=False, edge_excl=0, int_excl=0):
+        """Primary deconvolution (pymoresane/main.py:53-521); updates self.model and self.residual."""
+        logger.info("Starting...")
+        self._check_modes(all_on_gpu, decom_mode, conv_device, extraction_mode)
+        self._check_shapes()
+        prm = make_params(self.dirty_data_shape[0], self.psf_data_shape[0], subregion, scale_count, sigma_level,
+                          loop_gain, tolerance, accuracy, major_loop_miter, minor_loop_miter, conv_mode,
+                          enforce_positivity, edge_suppression, edge_offset, flux_threshold, neg_comp, edge_excl,
+                          int_excl)
+        self._run(False, 0, 0, prm)
+
+    def moresane_by_scale(self, start_scale=1, stop_scale=20, subregion=None, sigma_level=4, loop_gain=0.1,
+                          tolerance=0.75, accuracy=1e-6, major_loop_miter=100, minor_loop_miter=30, all_on_gpu=False,
+                          decom_mode="hip", core_count=1, conv_device="hip", conv_mode="linear",
+                          extraction_mode="hip", enforce_positivity=False, edge_suppression=False, edge_offset=0,
+                          flux_threshold=0, neg_comp=False, edge_excl=0, int_excl=0):
+        """Scale-by-scale deconvolution (pymoresane/main.py:523-599)."""
+        self._check_modes(all_on_gpu, decom_mode, conv_device, extraction_mode)
+        self._check_shapes()
+        prm = make_params(self.dirty_data_shape[0], self.psf_data_shape[0], subregion, None, sigma_level, loop_gain,
+                          tolerance, accuracy, major_loop_miter, minor_loop_miter, conv_mode, enforce_positivity,
+                          edge_suppression, edge_offset, flux_threshold, neg_comp, edge_excl, int_excl)
+        self._run(True, start_scale, stop_scale, prm)
+        self.complete = False

@@ -1,0 +1,277 @@
+"""HIP path vs the float64 oracle, through the C ABI (via the reference-shaped Python API).
+
+Tolerances: float32 bulk arithmetic against the float64 oracle -- relative
+L2 <= 1e-5 for single IUWT/FFT operations, <= 1e-4 for deconvolved models
+and residuals.  Integer/index results (masks, labels, medians of identical
+inputs, thresholded planes) are compared bit-exactly on identical inputs.
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import convolution64 as oconv
+from oracle import iuwt64 as oiuwt
+from oracle import moresane64 as omain
+from oracle import toolbox64 as otools
+from pymoresane_amd import iuwt, iuwt_convolution as conv, iuwt_toolbox as tools, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+# ------------------------------------------------------------------------------------------ IUWT
+@pytest.mark.parametrize("shape,scales,adjust", [
+    ((64, 64), 5, 0), ((64, 64), 5, 2), ((48, 80), 4, 1), ((40, 40), 4, 0), ((256, 256), 7, 0),
+    ((256, 256), 4, 3), ((96, 32), 4, 0), ((512, 512), 6, 0),
+])
+def test_iuwt_decomposition_matches_oracle(hip, shape, scales, adjust):
+    say("iuwt decomposition %s S=%d adj=%d" % (shape, scales, adjust))
+    x = np.random.default_rng(1).standard_normal(shape).astype(np.float32)
+    det, sm = iuwt.iuwt_decomposition(x, scales, adjust, store_smoothed=True)
+    rdet, rsm = oiuwt.iuwt_decomposition(x, scales, adjust, store_smoothed=True)
+    assert det.shape == rdet.shape and det.dtype == np.float32
+    for k in range(det.shape[0]):
+        assert rel(det[k], rdet[k]) < 1e-5, (k, rel(det[k], rdet[k]))
+    assert rel(sm, rsm) < 1e-6
+
+
+@pytest.mark.parametrize("shape,scales,adjust,smoothed", [
+    ((64, 64), 5, 0, True), ((64, 64), 5, 2, False), ((48, 80), 4, 1, True), ((256, 256), 6, 0, False),
+    ((32, 32), 1, 0, False), ((32, 32), 4, 3, False),
+])
+def test_iuwt_recomposition_matches_oracle(hip, shape, scales, adjust, smoothed):
+    say("iuwt recomposition %s S=%d adj=%d smoothed=%s" % (shape, scales, adjust, smoothed))
+    rng = np.random.default_rng(2)
+    det = rng.standard_normal((scales - adjust,) + shape).astype(np.float32)
+    sm = rng.standard_normal(shape).astype(np.float32) if smoothed else None
+    got = iuwt.iuwt_recomposition(det, adjust, smoothed_array=sm)
+    ref = oiuwt.iuwt_recomposition(det, adjust, sm)
+    assert rel(got, ref) < 1e-5
+
+
+def test_iuwt_round_trip_at_full_size(hip):
+    """4096^2, S=7, store_smoothed: decomposition then recomposition reproduces the input."""
+    say("iuwt round trip 4096^2 S=7")
+    img, _ = synth.config_c1(n=4096)
+    det, sm = iuwt.iuwt_decomposition(img, 7, store_smoothed=True, store_on_gpu=True)
+    rec = iuwt.iuwt_recomposition(det, 0, smoothed_array=sm)
+    assert rel(rec, img) < 1e-6
+    # linearity: W(2a) == 2 W(a) exactly in binary floating point
+    d2 = iuwt.iuwt_decomposition(2 * img[:1024, :1024], 5)
+    d1 = iuwt.iuwt_decomposition(img[:1024, :1024], 5)
+    np.testing.assert_array_equal(d2, 2 * d1)
+
+
+def test_iuwt_constant_image_and_device_inputs(hip):
+    say("iuwt constant image")
+    c = np.full((128, 128), 2.5, np.float32)
+    det, sm = iuwt.iuwt_decomposition(c, 6, store_smoothed=True)
+    assert np.max(np.abs(det)) < 1e-6
+    assert np.max(np.abs(sm - 2.5)) < 1e-6
+
+
+def test_iuwt_rejects_too_many_scales(hip):
+    with pytest.raises(Exception, match="2\\^\\(scale\\+1\\)"):
+        iuwt.iuwt_decomposition(np.zeros((16, 16), np.float32), 5)
+
+
+# ------------------------------------------------------------------------------------------- FFT
+@pytest.mark.parametrize("n,mode", [(4, "linear"), (8, "linear"), (16, "circular"), (64, "linear"),
+                                    (256, "linear"), (256, "circular"), (512, "linear"), (1024, "circular")])
+def test_fft_convolve_matches_oracle(hip, n, mode):
+    say("fft_convolve n=%d %s" % (n, mode))
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((n, n)).astype(np.float32)
+    p = 2 * n if mode == "linear" else n
+    psf = rng.standard_normal((p, p)).astype(np.float32)
+    got = conv.fft_convolve(x, conv.psf_spectrum(psf), conv_mode=mode)
+    ref = oconv.fft_convolve(x, oconv.psf_spectrum(psf), mode)
+    assert got.dtype == np.float32 and got.shape == (n, n)
+    assert rel(got, ref) < 2e-6, rel(got, ref)
+
+
+def test_fft_convolve_accepts_numpy_spectrum_and_padded_psf(hip):
+    say("fft_convolve numpy spectrum / padded psf")
+    rng = np.random.default_rng(4)
+    n = 128
+    x = rng.standard_normal((n, n)).astype(np.float32)
+    psf = synth.make_psf(n)  # same-size PSF: the linear path pads it (main.py:203-206)
+    ref = oconv.fft_convolve(x, oconv.psf_spectrum(oconv.pad_array(psf)), "linear")
+    got_dev = conv.fft_convolve(x, conv.psf_spectrum(psf, 2 * n), conv_mode="linear")
+    got_np = conv.fft_convolve(x, np.fft.rfft2(conv.pad_array(psf)), conv_mode="linear")
+    assert rel(got_dev, ref) < 2e-6 and rel(got_np, ref) < 2e-6
+    # gpu_r2c_fft round trip to the numpy layout
+    s = conv.gpu_r2c_fft(oconv.pad_array(psf).astype(np.float32))
+    assert rel(s, np.fft.rfft2(oconv.pad_array(psf))) < 2e-6
+
+
+def test_fft_convolve_full_size_linearity(hip):
+    """4096^2 image with the 8192^2 Gaussian PSF: conv(a+b) == conv(a)+conv(b), and a delta returns the PSF."""
+    say("fft_convolve 4096 linear, P=8192")
+    img, psf = synth.config_c1(n=4096)
+    spec = conv.psf_spectrum(psf)
+    n = 4096
+    delta = np.zeros((n, n), np.float32)
+    delta[1000, 3000] = 1.0
+    out = conv.fft_convolve(delta, spec, conv_mode="linear")
+    expect = psf[n - 1000:2 * n - 1000, n - 3000:2 * n - 3000]  # out[i] = psf[i - u + N]
+    assert np.max(np.abs(out - expect)) < 2e-6
+    a = img
+    b = np.roll(img, 17, axis=1)
+    ca = conv.fft_convolve(a, spec)
+    cb = conv.fft_convolve(b, spec)
+    cab = conv.fft_convolve(a + b, spec)
+    assert rel(cab, ca + cb) < 1e-6
+
+
+# ------------------------------------------------------------------------------------ threshold
+@pytest.mark.parametrize("shape,ee,ie", [((64, 64), 0, 0), ((63, 64), 0, 0), ((64, 64), 5, 0), ((64, 64), 0, 7),
+                                         ((64, 64), 3, 9), ((256, 256), 0, 0), ((1024, 1024), 0, 0)])
+def test_estimate_threshold_is_exact(hip, shape, ee, ie):
+    say("estimate_threshold %s edge=%d int=%d" % (shape, ee, ie))
+    rng = np.random.default_rng(5)
+    w = rng.standard_normal((3,) + shape).astype(np.float32)
+    w[1, :5, :5] = 0.0          # ties at zero
+    w[2] = np.round(w[2] * 4) / 4  # many ties
+    got = tools.estimate_threshold(w, ee, ie)
+    ref = otools.estimate_threshold(w, ee, ie)
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_apply_threshold_is_exact(hip):
+    say("apply_threshold")
+    rng = np.random.default_rng(6)
+    w = rng.standard_normal((4, 96, 96)).astype(np.float32)
+    thr = otools.estimate_threshold(w)
+    w[0, 0, 0] = np.float32(4 * thr[0])  # on the threshold: dropped (strict >)
+    got = tools.apply_threshold(w, thr, 4)
+    ref = otools.apply_threshold(w, thr, 4)
+    np.testing.assert_array_equal(got, ref.astype(np.float32))
+    two_d = tools.apply_threshold(w[1], thr[1], 3)
+    np.testing.assert_array_equal(two_d, otools.apply_threshold(w[1], thr[1], 3).astype(np.float32))
+
+
+# ----------------------------------------------------------------------------------- extraction
+def sparse_planes(seed, s, h, w, density=0.08):
+    rng = np.random.default_rng(seed)
+    v = rng.standard_normal((s, h, w)).astype(np.float32)
+    keep = rng.random((s, h, w)) < density
+    # grow blobs so objects span several pixels and scales overlap
+    from scipy import ndimage
+    keep = np.stack([ndimage.binary_dilation(k, iterations=1) for k in keep])
+    return np.where(keep, v, 0.0).astype(np.float32)
+
+
+@pytest.mark.parametrize("seed,shape,neg", [(0, (3, 64, 64), False), (1, (4, 128, 96), False),
+                                            (2, (2, 64, 64), True), (3, (1, 33, 47), False),
+                                            (4, (5, 256, 256), False)])
+def test_source_extraction_bit_exact(hip, seed, shape, neg):
+    say("source_extraction seed=%d %s neg=%s" % (seed, shape, neg))
+    w = sparse_planes(seed, *shape)
+    ext, mask, labels = tools.source_extraction(w, 0.5, neg_comp=neg, return_labels=True)
+    rext, rmask, rlabels = otools.source_extraction(w, 0.5, neg_comp=neg, return_labels=True)
+    np.testing.assert_array_equal(labels, rlabels)
+    np.testing.assert_array_equal(mask, rmask)
+    np.testing.assert_array_equal(ext, rext.astype(np.float32))
+
+
+def test_source_extraction_empty_and_full(hip):
+    say("source_extraction empty/full")
+    z = np.zeros((2, 32, 32), np.float32)
+    ext, mask = tools.source_extraction(z, 0.75)
+    assert not mask.any() and not ext.any()
+    f = np.ones((1, 32, 32), np.float32)
+    ext, mask = tools.source_extraction(f, 0.75)
+    assert mask.all()
+
+
+def test_snr_ratio(hip):
+    say("snr_ratio")
+    rng = np.random.default_rng(8)
+    a = rng.standard_normal((3, 64, 64)).astype(np.float32)
+    b = a + 0.01 * rng.standard_normal(a.shape).astype(np.float32)
+    assert tools.snr_ratio(a, b) == pytest.approx(otools.snr_ratio(a, b), rel=1e-12)
+
+
+# ------------------------------------------------------------------------------------ MORESANE
+def discrete(trace):
+    """Decision trace without the floating-point payloads."""
+    out = []
+    for e in trace:
+        if e[0] == "scales":
+            out.append(e[:5])
+        elif e[0] == "minor":
+            out.append(e[:4])
+        elif e[0] == "update":
+            out.append((e[0], e[1], e[4]))
+        else:
+            out.append(e)
+    return out
+
+
+def run_pair(dirty, psf, **kw):
+    from pymoresane_amd.main import DataImage
+    img = DataImage(dirty, psf, None, 1.0, 1.0)
+    img.moresane(**kw)
+    ref = omain.OracleImage(dirty, psf)
+    ref.moresane(**kw)
+    return img, ref
+
+
+def test_moresane_small_linear(hip):
+    say("moresane 64^2 linear")
+    dirty, psf = synth.make_pair(64, 21, n_points=8, n_blobs=1)
+    img, ref = run_pair(dirty, psf, scale_count=3, loop_gain=0.2, major_loop_miter=20, enforce_positivity=True)
+    assert discrete(img.trace) == discrete(ref.trace)
+    assert rel(img.model, ref.model) < 1e-4
+    assert rel(img.residual, ref.residual) < 1e-4
+
+
+def test_moresane_small_circular_options(hip):
+    say("moresane 64^2 circular + options")
+    dirty, psf = synth.make_pair(64, 22, psf_factor=1, n_points=8, n_blobs=1)
+    kw = dict(scale_count=4, loop_gain=0.3, major_loop_miter=15, conv_mode="circular", edge_suppression=True,
+              edge_excl=3, int_excl=4, neg_comp=True)
+    img, ref = run_pair(dirty, psf, **kw)
+    assert discrete(img.trace) == discrete(ref.trace)
+    assert rel(img.model, ref.model) < 1e-4
+    assert rel(img.residual, ref.residual) < 1e-4
+
+
+def test_moresane_config_c0(hip):
+    """256^2 dirty, 512^2 PSF, S=4, gain 0.2, sigma 4, 100 major, tol 0.75, positivity, linear."""
+    say("moresane c0")
+    dirty, psf = synth.config_c0()
+    kw = dict(scale_count=4, loop_gain=0.2, sigma_level=4, major_loop_miter=100, tolerance=0.75,
+              enforce_positivity=True, conv_mode="linear", minor_loop_miter=30, accuracy=1e-6)
+    img, ref = run_pair(dirty, psf, **kw)
+    say("c0 model rel %.3e residual rel %.3e" % (rel(img.model, ref.model), rel(img.residual, ref.residual)))
+    assert discrete(img.trace) == discrete(ref.trace)
+    assert rel(img.model, ref.model) < 1e-4
+    assert rel(img.residual, ref.residual) < 1e-4
+
+
+def test_moresane_by_scale_small(hip):
+    say("moresane_by_scale 64^2")
+    from pymoresane_amd.main import DataImage
+    dirty, psf = synth.make_pair(64, 23, n_points=10, n_blobs=2)
+    kw = dict(loop_gain=0.2, major_loop_miter=10, enforce_positivity=True)
+    img = DataImage(dirty, psf, None, 1.0, 1.0)
+    img.moresane_by_scale(start_scale=1, stop_scale=4, **kw)
+    ref = omain.OracleImage(dirty, psf)
+    ref.moresane_by_scale(start_scale=1, stop_scale=4, **kw)
+    assert discrete(img.trace) == discrete(ref.trace)
+    assert rel(img.model, ref.model) < 1e-4
+    assert rel(img.residual, ref.residual) < 1e-4
