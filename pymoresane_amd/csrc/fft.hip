@@ -158,33 +158,7 @@ __device__ __forceinline__ void run_stage(int lr, float2* sm, int L, int Ls, int
   }
 }
 
-// forward: natural order in LDS -> digit-reversed order in LDS (all groups in lockstep)
-__device__ __forceinline__ void fft_forward(float2* sm, int m, int lt, int nthr, const float2* tw) {
-  if (m <= 0) return;
-  const int ns = plan_stages(m);
-  const int L = 1 << m;
-  run_stage<true>(plan_first_log2(m), sm, L, L, lt, nthr, tw);
-  __syncthreads();
-  for (int i = 1; i < ns; ++i) {
-    run_stage<true>(4, sm, L, 1 << (4 * (ns - i)), lt, nthr, tw);
-    __syncthreads();
-  }
-}
-
-// inverse (unnormalised): digit-reversed order in LDS -> natural order in LDS
-__device__ __forceinline__ void fft_inverse(float2* sm, int m, int lt, int nthr, const float2* tw) {
-  if (m <= 0) return;
-  const int ns = plan_stages(m);
-  const int L = 1 << m;
-  for (int i = ns - 1; i >= 1; --i) {
-    run_stage<false>(4, sm, L, 1 << (4 * (ns - i)), lt, nthr, tw);
-    __syncthreads();
-  }
-  run_stage<false>(plan_first_log2(m), sm, L, L, lt, nthr, tw);
-  __syncthreads();
-}
-
-// storage position of frequency k after fft_forward of length 2^m:
+// storage position of frequency k after the forward transform of length 2^m:
 // k = d0 + R0 (d1 + 16 (d2 + ...)), position = d0 L/R0 + d1 L/(16 R0) + ...
 __host__ __device__ __forceinline__ int perm_pos(int k, int m) {
   const int ns = plan_stages(m);
@@ -206,7 +180,83 @@ __device__ __forceinline__ float frame_val(const float* src, int q, int off, int
   return (a < q && b < q) ? src[(size_t)a * q + b] : 0.f;
 }
 
+// ---- fused stage helpers --------------------------------------------------------------
+// First DIF stage (radix R0, sub-transform length L, M0 = L/R0) fused with the load of the
+// transform input: element e is fetched by get(e).  Batches of U butterflies are loaded before
+// any is computed so that ~16 global loads per thread are in flight.
+template <int R0, class Get>
+__device__ __forceinline__ void stage0_load(float2* sm, int L, int lt, int nthr, const float2* tw, Get get) {
+  const int M0 = L / R0;
+  const int step = kFftMaxLen / L;
+  constexpr int U = (R0 >= 16) ? 1 : (R0 >= 8) ? 2 : (16 / R0);
+  for (int n0 = lt; n0 < M0; n0 += nthr * U) {
+    float2 x[U][R0];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int n = n0 + u * nthr;
+#pragma unroll
+      for (int k = 0; k < R0; ++k) x[u][k] = (n < M0) ? get(n + k * M0) : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int n = n0 + u * nthr;
+      if (n < M0) {
+        Dft<R0, -1>::run(x[u]);
+        if (M0 > 1) {
+          float2 wk[R0];
+          powers<R0>(tw[n * step], wk);
+#pragma unroll
+          for (int k = 1; k < R0; ++k) x[u][k] = cmul(x[u][k], wk[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < R0; ++k) sm[pad(n + k * M0)] = x[u][k];
+      }
+    }
+  }
+}
+
+// Mirror of stage0_load for the inverse transform: the last inverse stage fused with the
+// store of the result; element e goes to put(e, value).
+template <int R0, class Put>
+__device__ __forceinline__ void stage0_store(const float2* sm, int L, int lt, int nthr, const float2* tw, Put put) {
+  const int M0 = L / R0;
+  const int step = kFftMaxLen / L;
+  for (int n = lt; n < M0; n += nthr) {
+    float2 x[R0];
+#pragma unroll
+    for (int k = 0; k < R0; ++k) x[k] = sm[pad(n + k * M0)];
+    if (M0 > 1) {
+      float2 wk[R0];
+      powers<R0>(cconj(tw[n * step]), wk);
+#pragma unroll
+      for (int k = 1; k < R0; ++k) x[k] = cmul(x[k], wk[k]);
+    }
+    Dft<R0, +1>::run(x);
+#pragma unroll
+    for (int k = 0; k < R0; ++k) put(n + k * M0, x[k]);
+  }
+}
+
+// middle radix-16 stages i in [i0, i1) (forward) of an ns-stage plan: Ls = 16^(ns-i)
+__device__ __forceinline__ void mid_forward(float2* sm, int L, int ns, int i0, int i1, int lt, int nthr,
+                                            const float2* tw) {
+  for (int i = i0; i < i1; ++i) {
+    fft_stage<16, true>(sm, L, 1 << (4 * (ns - i)), lt, nthr, tw);
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ void mid_inverse(float2* sm, int L, int ns, int i0, int i1, int lt, int nthr,
+                                            const float2* tw) {
+  for (int i = i1 - 1; i >= i0; --i) {
+    fft_stage<16, false>(sm, L, 1 << (4 * (ns - i)), lt, nthr, tw);
+    __syncthreads();
+  }
+}
+
 // ---- pass 1: rows r2c.  Group g of the block handles frame row fr; output T[k*ldT + fr], k in [0, P/2].
+// The frame row is the image row (off == 0, q == n) or the centred/rolled PSF (off = q/2).
+template <int R0>
 __global__ __launch_bounds__(kThreads) void k_rows_r2c(const float* __restrict__ src, int q, int off, int pm,
                                                        int n_rows, int rb, float2* __restrict__ T, int ldT,
                                                        const float2* __restrict__ tw, const int* pred) {
@@ -218,16 +268,29 @@ __global__ __launch_bounds__(kThreads) void k_rows_r2c(const float* __restrict__
   const int nthr = kThreads / rb;
   const int g = threadIdx.x / nthr;
   const int lt = threadIdx.x - g * nthr;
-  const int fr = blockIdx.x * rb + g;
+  const int fr = xcd_remap(blockIdx.x, gridDim.x) * rb + g;
   float2* sm = smem + g * (L + (L >> 5));
   const bool live = fr < n_rows;
-  for (int n = lt; n < L; n += nthr) {
-    float2 z = make_float2(0.f, 0.f);
-    if (live) z = make_float2(frame_val(src, q, off, P - 1, fr, 2 * n), frame_val(src, q, off, P - 1, fr, 2 * n + 1));
-    sm[pad(n)] = z;
+  const int pmask = P - 1;
+  const int a = (fr + off) & pmask;
+  const bool row_ok = live && a < q;
+  const float* row = src + (size_t)a * q;
+  if (off == 0) {
+    stage0_load<R0>(sm, L, lt, nthr, tw, [&](int e) {
+      const int c = 2 * e;
+      if (!row_ok || c >= q) return make_float2(0.f, 0.f);
+      if (c + 1 < q) return *reinterpret_cast<const float2*>(row + c);
+      return make_float2(row[c], 0.f);
+    });
+  } else {
+    stage0_load<R0>(sm, L, lt, nthr, tw, [&](int e) {
+      if (!row_ok) return make_float2(0.f, 0.f);
+      const int b0 = (2 * e + off) & pmask, b1 = (2 * e + 1 + off) & pmask;
+      return make_float2(b0 < q ? row[b0] : 0.f, b1 < q ? row[b1] : 0.f);
+    });
   }
   __syncthreads();
-  fft_forward(sm, mL, lt, nthr, tw);
+  mid_forward(sm, L, plan_stages(mL), 1, plan_stages(mL), lt, nthr, tw);
   if (!live) return;
   const int step = kFftMaxLen / P;
   for (int k = lt; k <= L; k += nthr) {
@@ -236,12 +299,15 @@ __global__ __launch_bounds__(kThreads) void k_rows_r2c(const float* __restrict__
     const float2 e = cscale(cadd(zk, zl), 0.5f);
     const float2 dd = cscale(csub(zk, zl), 0.5f);
     const float2 o = make_float2(dd.y, -dd.x);  // (zk - conj zl) / (2i)
-    const float2 w = tw[k * step];
-    T[(size_t)k * ldT + fr] = cadd(e, cmul(w, o));
+    T[(size_t)k * ldT + fr] = cadd(e, cmul(tw[k * step], o));
   }
 }
 
-// ---- pass 2a: column convolution.  Group g handles frequency column k.
+// ---- pass 2a: column convolution.  Group g handles frequency column k:
+// forward FFT (first stage fused with the load of the n_in stored rows, zero padding implicit),
+// last forward stage + PSF product + first inverse stage fused in registers, inverse FFT with
+// its last stage fused with the store of the n_out kept rows.
+template <int R0>
 __global__ __launch_bounds__(kThreads) void k_cols_conv(const float2* __restrict__ T1, int n_in, int pm,
                                                         const float2* __restrict__ S, float2* __restrict__ T2,
                                                         int n_out, int cpb, const float2* __restrict__ tw,
@@ -257,21 +323,43 @@ __global__ __launch_bounds__(kThreads) void k_cols_conv(const float2* __restrict
   const bool live = k < ncols;
   float2* sm = smem + g * (P + (P >> 5));
   const float2* col = T1 + (size_t)k * n_in;
-  for (int i = lt; i < P; i += nthr) sm[pad(i)] = (live && i < n_in) ? col[i] : make_float2(0.f, 0.f);
+  const float2* sc = S + (size_t)k * P;
+  stage0_load<R0>(sm, P, lt, nthr, tw,
+                  [&](int e) { return (live && e < n_in) ? col[e] : make_float2(0.f, 0.f); });
   __syncthreads();
-  fft_forward(sm, pm, lt, nthr, tw);
-  if (live) {
-    const float2* sc = S + (size_t)k * P;
-    for (int i = lt; i < P; i += nthr) sm[pad(i)] = cmul(sm[pad(i)], sc[i]);
+  const int ns = plan_stages(pm);
+  if (ns == 1) {
+    if (live)
+      for (int i = lt; i < P; i += nthr) sm[pad(i)] = cmul(sm[pad(i)], sc[i]);
+    __syncthreads();
+  } else {
+    mid_forward(sm, P, ns, 1, ns - 1, lt, nthr, tw);
+    for (int b = lt; b < (P >> 4); b += nthr) {
+      const int base = b << 4;
+      float2 x[16], s[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) s[j] = live ? sc[base + j] : make_float2(0.f, 0.f);
+#pragma unroll
+      for (int j = 0; j < 16; ++j) x[j] = sm[pad(base + j)];
+      Dft<16, -1>::run(x);
+#pragma unroll
+      for (int j = 0; j < 16; ++j) x[j] = cmul(x[j], s[j]);
+      Dft<16, +1>::run(x);
+#pragma unroll
+      for (int j = 0; j < 16; ++j) sm[pad(base + j)] = x[j];
+    }
+    __syncthreads();
+    mid_inverse(sm, P, ns, 1, ns - 1, lt, nthr, tw);
   }
-  __syncthreads();
-  fft_inverse(sm, pm, lt, nthr, tw);
   if (!live) return;
   float2* out = T2 + (size_t)k * n_out;
-  for (int i = lt; i < n_out; i += nthr) out[i] = sm[pad(i)];
+  stage0_store<R0>(sm, P, lt, nthr, tw, [&](int e, float2 v) {
+    if (e < n_out) out[e] = v;
+  });
 }
 
-// ---- pass 2b: column forward FFT only (PSF spectrum)
+// ---- pass 2b: column forward FFT only (PSF spectrum), stored in the digit-reversed order
+template <int R0>
 __global__ __launch_bounds__(kThreads) void k_cols_fwd(const float2* __restrict__ T1, int pm, float2* __restrict__ S,
                                                        int cpb, const float2* __restrict__ tw) {
   extern __shared__ float2 smem[];
@@ -284,15 +372,16 @@ __global__ __launch_bounds__(kThreads) void k_cols_fwd(const float2* __restrict_
   const bool live = k < ncols;
   float2* sm = smem + g * (P + (P >> 5));
   const float2* col = T1 + (size_t)k * P;
-  for (int i = lt; i < P; i += nthr) sm[pad(i)] = live ? col[i] : make_float2(0.f, 0.f);
+  stage0_load<R0>(sm, P, lt, nthr, tw, [&](int e) { return live ? col[e] : make_float2(0.f, 0.f); });
   __syncthreads();
-  fft_forward(sm, pm, lt, nthr, tw);
+  mid_forward(sm, P, plan_stages(pm), 1, plan_stages(pm), lt, nthr, tw);
   if (!live) return;
   float2* out = S + (size_t)k * P;
   for (int i = lt; i < P; i += nthr) out[i] = sm[pad(i)];
 }
 
 // ---- pass 3: rows c2r.  Group g handles kept row r; out[r * n_cols + c], c < n_cols.
+template <int R0>
 __global__ __launch_bounds__(kThreads) void k_rows_c2r(const float2* __restrict__ T2, int ldT, int pm, int n_rows,
                                                        int rb, float* __restrict__ out, int n_cols, float scale,
                                                        const float2* __restrict__ tw, const int* pred) {
@@ -304,40 +393,51 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r(const float2* __restrict_
   const int nthr = kThreads / rb;
   const int g = threadIdx.x / nthr;
   const int lt = threadIdx.x - g * nthr;
-  const int r = blockIdx.x * rb + g;
+  const int r = xcd_remap(blockIdx.x, gridDim.x) * rb + g;
   const bool live = r < n_rows;
   float2* sm = smem + g * (L + (L >> 5));
   const int step = kFftMaxLen / P;
-  for (int k = lt; k <= (L >> 1); k += nthr) {
-    float2 xk = make_float2(0.f, 0.f), xl = make_float2(0.f, 0.f);
-    if (live) {
-      xk = T2[(size_t)k * ldT + r];
-      xl = T2[(size_t)(L - k) * ldT + r];
+  const int half = L >> 1;
+  constexpr int U = 8;
+  for (int k0 = lt; k0 <= half; k0 += nthr * U) {
+    float2 xk[U], xl[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u * nthr;
+      const bool ok = live && k <= half;
+      xk[u] = ok ? T2[(size_t)k * ldT + r] : make_float2(0.f, 0.f);
+      xl[u] = ok ? T2[(size_t)(L - k) * ldT + r] : make_float2(0.f, 0.f);
     }
-    {
-      // Z'[k] = (X[k] + conj X[L-k]) + i (X[k] - conj X[L-k]) conj(w^k)
-      const float2 e = cadd(xk, cconj(xl));
-      const float2 o = cmul(csub(xk, cconj(xl)), cconj(tw[k * step]));
-      sm[pad(perm_pos(k & (L - 1), mL))] = make_float2(e.x - o.y, e.y + o.x);
-    }
-    const int kl = L - k;
-    if (kl != k && kl < L) {
-      const float2 e = cadd(xl, cconj(xk));
-      const float2 o = cmul(csub(xl, cconj(xk)), cconj(tw[kl * step]));
-      sm[pad(perm_pos(kl, mL))] = make_float2(e.x - o.y, e.y + o.x);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u * nthr;
+      if (k > half) break;
+      {
+        // Z'[k] = (X[k] + conj X[L-k]) + i (X[k] - conj X[L-k]) conj(w^k)
+        const float2 e = cadd(xk[u], cconj(xl[u]));
+        const float2 o = cmul(csub(xk[u], cconj(xl[u])), cconj(tw[k * step]));
+        sm[pad(perm_pos(k & (L - 1), mL))] = make_float2(e.x - o.y, e.y + o.x);
+      }
+      const int kl = L - k;
+      if (kl != k && kl < L) {
+        const float2 e = cadd(xl[u], cconj(xk[u]));
+        const float2 o = cmul(csub(xl[u], cconj(xk[u])), cconj(tw[kl * step]));
+        sm[pad(perm_pos(kl, mL))] = make_float2(e.x - o.y, e.y + o.x);
+      }
     }
   }
   __syncthreads();
-  fft_inverse(sm, mL, lt, nthr, tw);
+  mid_inverse(sm, L, plan_stages(mL), 1, plan_stages(mL), lt, nthr, tw);
   if (!live) return;
   float* orow = out + (size_t)r * n_cols;
-  for (int n = lt; n < L; n += nthr) {
-    const int c = 2 * n;
-    if (c >= n_cols) break;
-    const float2 z = sm[pad(n)];
-    orow[c] = z.x * scale;
-    if (c + 1 < n_cols) orow[c + 1] = z.y * scale;
-  }
+  stage0_store<R0>(sm, L, lt, nthr, tw, [&](int e, float2 v) {
+    const int c = 2 * e;
+    if (c + 1 < n_cols) {
+      *reinterpret_cast<float2*>(orow + c) = make_float2(v.x * scale, v.y * scale);
+    } else if (c < n_cols) {
+      orow[c] = v.x * scale;
+    }
+  });
 }
 
 struct DeviceTwiddles {
@@ -346,8 +446,14 @@ struct DeviceTwiddles {
 std::mutex g_tw_mu;
 DeviceTwiddles g_tw[64];
 
+int env_int(const char* name, int dflt) {
+  const char* v = getenv(name);
+  return v ? atoi(v) : dflt;
+}
+
 int rows_per_block(int L) {
-  int rb = 8192 / (L > 0 ? L : 1);
+  static const int lim = env_int("PMS_FFT_ROW_LDS", 8192);
+  int rb = lim / (L > 0 ? L : 1);
   if (rb < 1) rb = 1;
   if (rb > 16) rb = 16;
   return rb;
@@ -358,6 +464,45 @@ int cols_per_block(int P) {
   if (c < 1) c = 1;
   if (c > 16) c = 16;
   return c;
+}
+
+template <int R0>
+void set_lds_attrs(int lds_max) {
+  (void)hipFuncSetAttribute((const void*)k_rows_r2c<R0>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+  (void)hipFuncSetAttribute((const void*)k_cols_conv<R0>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+  (void)hipFuncSetAttribute((const void*)k_cols_fwd<R0>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+  (void)hipFuncSetAttribute((const void*)k_rows_c2r<R0>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+}
+
+// first-stage radix of the plan for length 2^m
+int first_radix(int m) { return 1 << plan_first_log2(m); }
+
+#define PMS_FFT_DISPATCH(R, KERNEL, ...)                                          \
+  switch (R) {                                                                    \
+    case 2: hipLaunchKernelGGL(KERNEL<2>, __VA_ARGS__); break;                    \
+    case 4: hipLaunchKernelGGL(KERNEL<4>, __VA_ARGS__); break;                    \
+    case 8: hipLaunchKernelGGL(KERNEL<8>, __VA_ARGS__); break;                    \
+    default: hipLaunchKernelGGL(KERNEL<16>, __VA_ARGS__); break;                  \
+  }
+
+void launch_rows_r2c(const float* src, int q, int off, int pm, int n_rows, float2* T, int ldT, const float2* tw,
+                     const int* pred, hipStream_t st) {
+  const int L = (1 << pm) / 2;
+  const int rb = rows_per_block(L);
+  const size_t lds = (size_t)rb * (L + (L >> 5)) * sizeof(float2);
+  PMS_FFT_DISPATCH(first_radix(pm - 1), k_rows_r2c, dim3(ceil_div(n_rows, rb)), dim3(kThreads), lds, st, src, q, off,
+                   pm, n_rows, rb, T, ldT, tw, pred);
+  PMS_LAUNCH_CHECK();
+}
+
+void launch_rows_c2r(const float2* T2, int ldT, int pm, int n_rows, float* out, int n_cols, float scale,
+                     const float2* tw, const int* pred, hipStream_t st) {
+  const int L = (1 << pm) / 2;
+  const int rb = rows_per_block(L);
+  const size_t lds = (size_t)rb * (L + (L >> 5)) * sizeof(float2);
+  PMS_FFT_DISPATCH(first_radix(pm - 1), k_rows_c2r, dim3(ceil_div(n_rows, rb)), dim3(kThreads), lds, st, T2, ldT,
+                   pm, n_rows, rb, out, n_cols, scale, tw, pred);
+  PMS_LAUNCH_CHECK();
 }
 
 }  // namespace
@@ -373,10 +518,10 @@ void fft_init_device(int device) {
   }
   // allow the FFT kernels the large dynamic LDS (67.6 KB at P = 8192) that CDNA4's 160 KB LDS permits
   const int lds_max = 160 * 1024;
-  (void)hipFuncSetAttribute((const void*)k_rows_r2c, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
-  (void)hipFuncSetAttribute((const void*)k_cols_conv, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
-  (void)hipFuncSetAttribute((const void*)k_cols_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
-  (void)hipFuncSetAttribute((const void*)k_rows_c2r, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+  set_lds_attrs<2>(lds_max);
+  set_lds_attrs<4>(lds_max);
+  set_lds_attrs<8>(lds_max);
+  set_lds_attrs<16>(lds_max);
   (void)hipGetLastError();
   float2* d = nullptr;
   PMS_HIP(hipMalloc(&d, sizeof(float2) * kFftMaxLen));
@@ -414,15 +559,11 @@ void fft_psf_spectrum(const float* psf, int q, int p, float2* spec, float2* work
   const int pm = ilog2(p);
   const int L = p / 2;
   // work holds the (P/2+1) x P row spectra, column-major
-  const int rb = rows_per_block(L);
-  const size_t lds = (size_t)rb * (L + (L >> 5)) * sizeof(float2);
-  hipLaunchKernelGGL(k_rows_r2c, dim3(ceil_div(p, rb)), dim3(kThreads), lds, st, psf, q, q / 2, pm, p, rb, work,
-                     p, tw, (const int*)nullptr);
-  PMS_LAUNCH_CHECK();
+  launch_rows_r2c(psf, q, q / 2, pm, p, work, p, tw, nullptr, st);
   const int cpb = cols_per_block(p);
   const size_t lds2 = (size_t)cpb * (p + (p >> 5)) * sizeof(float2);
-  hipLaunchKernelGGL(k_cols_fwd, dim3(ceil_div(L + 1, cpb)), dim3(kThreads), lds2, st, (const float2*)work, pm,
-                     spec, cpb, tw);
+  PMS_FFT_DISPATCH(first_radix(pm), k_cols_fwd, dim3(ceil_div(L + 1, cpb)), dim3(kThreads), lds2, st,
+                   (const float2*)work, pm, spec, cpb, tw);
   PMS_LAUNCH_CHECK();
 }
 
@@ -435,20 +576,14 @@ void fft_conv(const float* in, int n, int p, const float2* spec, float* out, flo
   const int L = p / 2;
   float2* T1 = work;
   float2* T2 = work + (size_t)(L + 1) * n;
-  const int rb = rows_per_block(L);
-  const size_t lds = (size_t)rb * (L + (L >> 5)) * sizeof(float2);
-  hipLaunchKernelGGL(k_rows_r2c, dim3(ceil_div(n, rb)), dim3(kThreads), lds, st, in, n, 0, pm, n, rb, T1, n, tw,
-                     pred);
-  PMS_LAUNCH_CHECK();
+  launch_rows_r2c(in, n, 0, pm, n, T1, n, tw, pred, st);
   const int cpb = cols_per_block(p);
   const size_t lds2 = (size_t)cpb * (p + (p >> 5)) * sizeof(float2);
-  hipLaunchKernelGGL(k_cols_conv, dim3(ceil_div(L + 1, cpb)), dim3(kThreads), lds2, st, (const float2*)T1, n, pm,
-                     spec, T2, n, cpb, tw, pred);
+  PMS_FFT_DISPATCH(first_radix(pm), k_cols_conv, dim3(ceil_div(L + 1, cpb)), dim3(kThreads), lds2, st,
+                   (const float2*)T1, n, pm, spec, T2, n, cpb, tw, pred);
   PMS_LAUNCH_CHECK();
   const float scale = 1.0f / ((float)p * (float)p);
-  hipLaunchKernelGGL(k_rows_c2r, dim3(ceil_div(n, rb)), dim3(kThreads), lds, st, (const float2*)T2, n, pm, n, rb,
-                     out, n, scale, tw, pred);
-  PMS_LAUNCH_CHECK();
+  launch_rows_c2r((const float2*)T2, n, pm, n, out, n, scale, tw, pred, st);
 }
 
 // numpy rfft2 spectrum (rows = axis-0 frequency m in [0,P), cols = k in [0,P/2]) of an

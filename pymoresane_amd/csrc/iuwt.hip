@@ -233,11 +233,25 @@ int env_int(const char* name, int dflt) {
   return v ? atoi(v) : dflt;
 }
 
+// tuning knobs, read once per process
+struct IuwtTuning {
+  int qmax, krows, pf, target_blocks;
+  IuwtTuning()
+      : qmax(env_int("PMS_IUWT_QMAX", 16)),
+        krows(env_int("PMS_IUWT_KROWS", 0)),
+        pf(env_int("PMS_IUWT_PF", 3)),
+        target_blocks(env_int("PMS_IUWT_BLOCKS", 2048)) {}
+};
+const IuwtTuning& tuning() {
+  static const IuwtTuning t;
+  return t;
+}
+
 Plan make_plan(int h, int w, int scale, int halo) {
   Plan pl;
   const int d = 1 << scale;
   const int teeth = ceil_div(w, d);
-  const int qmax = env_int("PMS_IUWT_QMAX", 16);
+  const int qmax = tuning().qmax;
   int q = d < qmax ? d : qmax;
   int tj = kThreads / q;
   int tp = 1;
@@ -249,9 +263,9 @@ Plan make_plan(int h, int w, int scale, int halo) {
   const int mr = ceil_div(h, d);
   // aim for enough workgroups to fill 256 CUs several times over
   const int strips = d * ceil_div(d, q) * ceil_div(teeth, tj);
-  int k = env_int("PMS_IUWT_KROWS", 0);
+  int k = tuning().krows;
   if (k <= 0) {
-    const long target = 2048;
+    const long target = tuning().target_blocks;
     long segs = (target + strips - 1) / strips;
     if (segs < 1) segs = 1;
     k = ceil_div(mr, (int)segs);
@@ -270,14 +284,18 @@ Plan make_plan(int h, int w, int scale, int halo) {
 
 template <int MODE, int ML>
 void launch_pf(ScaleParams& sp, const Plan& pl, hipStream_t st) {
-  const int pf = env_int("PMS_IUWT_PF", 3);
+  const int pf = tuning().pf;
   dim3 grid(pl.blocks), block(kThreads);
   if (pf <= 1)
     hipLaunchKernelGGL((k_iuwt_scale<MODE, 1, ML>), grid, block, pl.lds_bytes, st, sp);
   else if (pf == 2)
     hipLaunchKernelGGL((k_iuwt_scale<MODE, 2, ML>), grid, block, pl.lds_bytes, st, sp);
-  else
+  else if (pf == 3)
     hipLaunchKernelGGL((k_iuwt_scale<MODE, 3, ML>), grid, block, pl.lds_bytes, st, sp);
+  else if (pf <= 4)
+    hipLaunchKernelGGL((k_iuwt_scale<MODE, 4, ML>), grid, block, pl.lds_bytes, st, sp);
+  else
+    hipLaunchKernelGGL((k_iuwt_scale<MODE, 6, ML>), grid, block, pl.lds_bytes, st, sp);
   PMS_LAUNCH_CHECK();
 }
 
