@@ -59,6 +59,90 @@ def cpu_baseline(sample_n=2048, scales=7):
                       % (sample_n, sample_n, scales, 2 * sample_n, dt, ratio)}
 
 
+MORESANE_KW = dict(loop_gain=0.2, sigma_level=4, major_loop_miter=100, tolerance=0.75, enforce_positivity=True,
+                   conv_mode="linear", minor_loop_miter=30, accuracy=1e-6)
+
+
+def pct(vals, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(vals), q)) if len(vals) else None
+
+
+def bench_batched(workers=16):
+    """96 ragged pairs (32x256^2, 32x512^2, 24x1024^2, 8x2048^2), S=5: plain (one worker) vs batched."""
+    import numpy as np
+    from pymoresane_amd import batch, synth
+    sides = synth.config_c3_sides()
+    log("c3: generating %d pairs" % len(sides))
+    pairs = [synth.config_c3_pair(i, s) for i, s in enumerate(sides)]
+    kw = dict(MORESANE_KW, scale_count=5)
+    jobs = batch.DeviceJobs(pairs, [kw] * len(pairs))
+    del pairs
+    order = sorted(range(len(sides)), key=lambda i: -sides[i])  # largest first
+    # plain: one job at a time on one stream
+    t0 = time.perf_counter()
+    plain_stats = batch.run_batch(jobs, n_workers=1, order=order)
+    t_plain = time.perf_counter() - t0
+    ref = jobs.results()
+    jobs.reset()
+    t0 = time.perf_counter()
+    stats = batch.run_batch(jobs, n_workers=workers, order=order)
+    t_batch = time.perf_counter() - t0
+    same = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) for a, b in zip(jobs.results(), ref))
+    lat = [s[2] for s in stats]
+    log("c3: plain %.2fs batched %.2fs (x%.2f) identical=%s" % (t_plain, t_batch, t_plain / t_batch, same))
+    return {"workload": "c3: 96 ragged pairs 32x256^2+32x512^2+24x1024^2+8x2048^2, PSF 2N, S=5",
+            "plain_images_per_s": round(len(sides) / t_plain, 3),
+            "batched_images_per_s": round(len(sides) / t_batch, 3),
+            "speedup": round(t_plain / t_batch, 3), "workers": workers, "identical_to_alone": bool(same),
+            "batched_job_p50_s": round(pct(lat, 50), 4), "batched_job_p99_s": round(pct(lat, 99), 4)}
+
+
+def bench_served(count=60, workers=16, rate=20.0, budget=200e9):
+    """Poisson arrivals at `rate`/s, sides 512/1024/2048/4096 (50/30/15/5 %), S = log2(side) - 5."""
+    import numpy as np
+    from pymoresane_amd import batch, synth
+    reqs = synth.config_c4_requests(count, rate=rate)
+    log("c4: generating %d requests" % count)
+    psf_cache = {}
+    inputs = []
+    for t_arr, side, seed in reqs:
+        rng = np.random.default_rng(seed)
+        if side not in psf_cache:
+            psf_cache[side] = synth.make_psf(2 * side)
+        sky = synth.make_sky(side, rng)
+        dirty = (synth.linear_convolve(sky, psf_cache[side]) + rng.normal(0.0, 1e-3, (side, side))).astype(np.float32)
+        inputs.append((t_arr, side, dirty))
+    out = {}
+    results = {}
+    for label, nw in (("plain", 1), ("served", workers)):
+        srv = batch.Server(n_workers=nw, budget_bytes=budget)
+        handles = []
+        t0 = time.perf_counter()
+        for t_arr, side, dirty in inputs:
+            wait = t_arr - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            handles.append(srv.submit(dirty, psf_cache[side], scale_count=int(np.log2(side)) - 5, **MORESANE_KW))
+        srv.drain()
+        span = time.perf_counter() - t0
+        lat = [h["req"].t_done - h["req"].t_submit for h in handles]
+        ok = all(h["req"].status == 0 for h in handles)
+        results[label] = [(h["model"], h["resid"]) for h in handles]
+        srv.close()
+        out[label] = {"images_per_s": round(len(handles) / span, 3), "p50_latency_s": round(pct(lat, 50), 4),
+                      "p99_latency_s": round(pct(lat, 99), 4), "makespan_s": round(span, 3), "workers": nw,
+                      "all_ok": ok}
+        log("c4 %s: %.2f img/s p50 %.3fs p99 %.3fs" % (label, len(handles) / span, pct(lat, 50), pct(lat, 99)))
+    same = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+               for a, b in zip(results["plain"], results["served"]))
+    out["identical_to_alone"] = bool(same)
+    out["speedup_images_per_s"] = round(out["served"]["images_per_s"] / out["plain"]["images_per_s"], 3)
+    out["workload"] = ("c4: %d Poisson arrivals at %.0f/s, sides 512/1024/2048/4096 (50/30/15/5%%), "
+                       "S=log2(side)-5, HBM budget %.0f GB" % (count, rate, budget / 1e9))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +151,11 @@ def main():
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--scales", type=int, default=7)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched (c3) and served (c4) forms")
+    ap.add_argument("--serve-requests", type=int, default=60)
+    ap.add_argument("--workers", type=int, default=16)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 256 MiB float32 copy kernel 5 times (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the MORESANE hot path runs one image per device and this build "
@@ -79,6 +168,18 @@ def main():
     from pymoresane_amd._lib import DeviceArray, Event, check
 
     lib = _lib.lib()
+    if args.calibrate_copy:
+        count = 64 << 20
+        a, b2 = DeviceArray((count,)), DeviceArray((count,))
+        a.fill(1.0)
+        e0, e1 = Event(), Event()
+        e0.record()
+        for _ in range(5):
+            check(lib.pms_copy_f32(b2.ptr, a.ptr, count, None))
+        e1.record()
+        ms = e0.elapsed_ms(e1) / 5
+        print(json.dumps({"copy_bytes_each_way": 4 * count, "ms": ms, "GBps_read_plus_write": gbs(8 * count, ms)}))
+        return
     n, S = args.n, args.scales
     p = 2 * n
     log("generating c1 inputs %dx%d, PSF %dx%d" % (n, n, p, p))
@@ -182,10 +283,15 @@ def main():
         "wall_ms_per_step": round(wall_ms, 4),
         "recon_rel_err": recon_err,
     }
+    check(lib.pms_stream_destroy(st))
+    for buf in (d_img, d_psf, det, sm, rec, out, spec, iw, fw, fw_psf):
+        buf.free()
+    if not args.no_batch:
+        res["batched"] = bench_batched(args.workers)
+        res["served"] = bench_served(args.serve_requests, args.workers)
     if not args.no_cpu:
         log("cpu baseline (float64 oracle, 1 core, 2048^2 sample)")
         res["cpu_baseline"] = cpu_baseline()
-    check(lib.pms_stream_destroy(st))
     print(json.dumps(res))
 
 

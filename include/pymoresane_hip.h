@@ -44,6 +44,9 @@ int pms_memcpy_h2d(void* dst, const void* src_host, size_t bytes, void* stream);
 int pms_memcpy_d2h(void* dst_host, const void* src, size_t bytes, void* stream);
 int pms_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
 int pms_fill_f32(float* ptr, size_t count, float value, void* stream);
+/* dst[i] = src[i] for count floats with a plain streaming kernel (also the
+ * reference point for calibrating HBM traffic counters). */
+int pms_copy_f32(float* dst, const float* src, size_t count, void* stream);
 int pms_stream_create(void** stream);
 int pms_stream_destroy(void* stream);
 int pms_stream_synchronize(void* stream);
@@ -196,6 +199,35 @@ typedef struct pms_batch_job {
 } pms_batch_job;
 
 int pms_batch_run(pms_batch_job* jobs, int n_jobs, int n_workers);
+
+/* ------------------------------------------------------------- served form */
+/* Continuous-batching server: requests queue FIFO and are admitted, as soon
+ * as a worker is free and their device footprint fits the HBM budget, onto
+ * one of n_workers streams, overlapping with requests already running.  Host
+ * buffers must stay valid until the request is done.  Times are seconds since
+ * the server was created.  No reference counterpart: the reference has no
+ * serving path. */
+typedef struct pms_serve_request {
+  const float* dirty_host;  /* n x n */
+  const float* psf_host;    /* psf_n x psf_n */
+  const float* mask_host;   /* n x n or NULL */
+  float* model_host;        /* out, n x n */
+  float* residual_host;     /* out, n x n */
+  pms_moresane_params prm;
+  int by_scale, start_scale, stop_scale;
+  int status;               /* 1 queued, 2 running, 0 done, < 0 error */
+  int major_iters;
+  int pad;
+  size_t footprint;         /* out: device bytes reserved while running */
+  double t_submit, t_start, t_done;
+} pms_serve_request;
+
+int pms_server_create(int n_workers, double budget_bytes, void** server);
+int pms_server_submit(void* server, pms_serve_request* req);
+int pms_server_wait(void* server, pms_serve_request* req, double timeout_s, int* done);
+int pms_server_drain(void* server);
+int pms_server_now(void* server, double* t);
+int pms_server_destroy(void* server);
 
 #ifdef __cplusplus
 }

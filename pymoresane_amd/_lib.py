@@ -48,6 +48,13 @@ class BatchJob(ctypes.Structure):
                 ("seconds", c_double), ("t_start", c_double)]
 
 
+class ServeRequest(ctypes.Structure):
+    _fields_ = [("dirty_host", c_void_p), ("psf_host", c_void_p), ("mask_host", c_void_p), ("model_host", c_void_p),
+                ("residual_host", c_void_p), ("prm", MoresaneParams), ("by_scale", c_int), ("start_scale", c_int),
+                ("stop_scale", c_int), ("status", c_int), ("major_iters", c_int), ("pad", c_int),
+                ("footprint", c_size_t), ("t_submit", c_double), ("t_start", c_double), ("t_done", c_double)]
+
+
 EV_SCALES, EV_MINOR, EV_UPDATE, EV_NOCOEFF, EV_POOR, EV_NOWORK, EV_BYSCALE = 1, 2, 3, 4, 5, 6, 7
 
 # name -> (restype, argtypes); mirrors include/pymoresane_hip.h one to one
@@ -62,6 +69,7 @@ SIGNATURES = {
     "pms_memcpy_d2h": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
     "pms_memcpy_d2d": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
     "pms_fill_f32": (c_int, [c_void_p, c_size_t, c_float, c_void_p]),
+    "pms_copy_f32": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
     "pms_stream_create": (c_int, [ctypes.POINTER(c_void_p)]),
     "pms_stream_destroy": (c_int, [c_void_p]),
     "pms_stream_synchronize": (c_int, [c_void_p]),
@@ -90,6 +98,12 @@ SIGNATURES = {
     "pms_moresane_by_scale": (c_int, [ctypes.POINTER(MoresaneParams), c_int, c_int, c_void_p, c_void_p, c_void_p,
                                       c_void_p, c_void_p, ctypes.POINTER(TraceEvent), c_int, c_int_p, c_void_p]),
     "pms_batch_run": (c_int, [ctypes.POINTER(BatchJob), c_int, c_int]),
+    "pms_server_create": (c_int, [c_int, c_double, ctypes.POINTER(c_void_p)]),
+    "pms_server_submit": (c_int, [c_void_p, ctypes.POINTER(ServeRequest)]),
+    "pms_server_wait": (c_int, [c_void_p, ctypes.POINTER(ServeRequest), c_double, c_int_p]),
+    "pms_server_drain": (c_int, [c_void_p]),
+    "pms_server_now": (c_int, [c_void_p, ctypes.POINTER(c_double)]),
+    "pms_server_destroy": (c_int, [c_void_p]),
 }
 
 _lock = threading.Lock()

@@ -150,6 +150,13 @@ int pms_fill_f32(float* ptr, size_t count, float value, void* stream) {
   });
 }
 
+int pms_copy_f32(float* dst, const float* src, size_t count, void* stream) {
+  return guarded([&] {
+    PMS_REQUIRE(dst && src, "NULL buffer");
+    if (count) copy_plane(src, dst, count, nullptr, S(stream));
+  });
+}
+
 int pms_stream_create(void** stream) {
   return guarded([&] {
     hipStream_t s = nullptr;

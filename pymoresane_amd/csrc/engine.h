@@ -45,6 +45,14 @@ struct Geometry {
 
 Geometry make_geometry(int n, int psf_n, const MoresaneParams& prm, int scale_count_in, int* subregion_out);
 
+// byte sizes of the per-job device buffers
+struct JobSizes {
+  size_t s2, S_s2, mask, iuwt, N2, spec_sub, spec_full, fft, psf, ext, partial, pmax;
+};
+
+// device bytes one job of this geometry needs (its workspace plus nothing else)
+size_t job_footprint_bytes(const Geometry& g);
+
 // Device buffers needed by one moresane() call, kept across calls of the same shape.
 class Job {
  public:
@@ -68,7 +76,7 @@ class Job {
   DeviceArena arena_;
   DeviceArena by_scale_arena_;
   Geometry geo_{};
-  int cap_S_ = 0;
+  JobSizes cap_{};
   bool have_ = false;
 
  public:
@@ -79,7 +87,7 @@ class Job {
         *cbuf = nullptr;
   float* iuwt_work = nullptr;
   float *model_local = nullptr, *resid = nullptr, *conv_full = nullptr, *psf_frame = nullptr;
-  float2 *spec_sub = nullptr, *spec_full = nullptr, *fft_work = nullptr;
+  float2 *spec_sub = nullptr, *spec_full = nullptr, *spec_full_buf = nullptr, *fft_work = nullptr;
   void *mad_work = nullptr, *ext_work = nullptr;
   double* partial = nullptr;
   float* pmax = nullptr;

@@ -185,54 +185,97 @@ Geometry make_geometry(int n, int psf_n, const MoresaneParams& prm, int scale_co
 Job::Job(hipStream_t st) : st_(st) {}
 Job::~Job() {}
 
-void Job::ensure(const Geometry& g) {
-  if (have_ && g.n == geo_.n && g.psf_n == geo_.psf_n && g.sub == geo_.sub && g.S <= cap_S_ &&
-      g.p_sub == geo_.p_sub && g.p_full == geo_.p_full) {
-    const int S_keep = cap_S_;
-    geo_ = g;
-    cap_S_ = S_keep;
-    return;
-  }
-  arena_.release();
-  geo_ = g;
-  cap_S_ = std::max(g.S, (int)std::log2((double)g.n) - 1);
-  const int S = cap_S_;
+// Byte sizes of every per-job buffer for geometry g (S = scale capacity).
+static JobSizes job_sizes(const Geometry& g, int S) {
+  JobSizes z{};
   const size_t N2 = (size_t)g.n * g.n;
   const size_t s2 = (size_t)g.sub * g.sub;
-  auto F = [&](size_t n) { return (float*)arena_.alloc(n * sizeof(float)); };
-  dirty_sub = F(s2);
-  psf_sub = F(s2);
-  D = F(S * s2);
-  Dcg = F(S * s2);
-  E = F(S * s2);
-  M = (uint8_t*)arena_.alloc(S * s2);
-  rec = F(s2);
-  x = F(s2);
-  xn = F(s2);
-  r = F(s2);
-  rn = F(s2);
-  p = F(s2);
-  ap = F(s2);
-  cbuf = F(s2);
-  iuwt_work = F(2 * s2);
-  model_local = F(N2);
-  resid = F(N2);
-  conv_full = F(N2);
-  spec_sub = (float2*)arena_.alloc(fft_spectrum_elems(g.p_sub) * sizeof(float2));
-  spec_full = (g.p_full == g.p_sub && g.sub == g.n) ? spec_sub
-                                                     : (float2*)arena_.alloc(fft_spectrum_elems(g.p_full) * sizeof(float2));
-  const size_t fw = std::max({fft_conv_work_elems(g.sub, g.p_sub), fft_conv_work_elems(g.n, g.p_full),
-                              fft_spectrum_elems(g.p_sub), fft_spectrum_elems(g.p_full)});
-  fft_work = (float2*)arena_.alloc(fw * sizeof(float2));
-  psf_frame = F((size_t)g.psf_n * g.psf_n);
-  mad_work = arena_.alloc(mad_work_bytes(S));
-  mad_work_init(mad_work, S, st_);
-  ext_work = arena_.alloc(extract_work_bytes(S, g.sub, g.sub));
-  extract_work_init(ext_work, S, g.sub, g.sub, st_);
-  const long np = std::max<long>({(long)iuwt_decompose_partials(g.sub, g.sub, S, 0), (long)reduce_grid(N2) * 2,
-                                  (long)reduce_grid(s2) * S, 4096L});
-  partial = (double*)arena_.alloc(np * sizeof(double));
-  pmax = (float*)arena_.alloc((size_t)S * 2048 * sizeof(float));
+  z.s2 = s2 * sizeof(float);
+  z.S_s2 = (size_t)S * s2 * sizeof(float);
+  z.mask = (size_t)S * s2;
+  z.iuwt = 2 * s2 * sizeof(float);
+  z.N2 = N2 * sizeof(float);
+  z.spec_sub = fft_spectrum_elems(g.p_sub) * sizeof(float2);
+  z.spec_full = fft_spectrum_elems(g.p_full) * sizeof(float2);
+  z.fft = std::max({fft_conv_work_elems(g.sub, g.p_sub), fft_conv_work_elems(g.n, g.p_full),
+                    fft_spectrum_elems(g.p_sub), fft_spectrum_elems(g.p_full)}) *
+          sizeof(float2);
+  z.psf = (size_t)g.psf_n * g.psf_n * sizeof(float);
+  z.ext = extract_work_bytes(S, g.sub, g.sub);
+  z.partial = (size_t)std::max<long>({(long)iuwt_decompose_partials(g.sub, g.sub, S, 0),
+                                      (long)reduce_grid(N2) * 2, (long)reduce_grid(s2) * S, 4096L}) *
+              sizeof(double);
+  z.pmax = (size_t)S * 2048 * sizeof(float);
+  return z;
+}
+
+static bool fits(const JobSizes& need, const JobSizes& have) {
+  return need.s2 <= have.s2 && need.S_s2 <= have.S_s2 && need.mask <= have.mask && need.iuwt <= have.iuwt &&
+         need.N2 <= have.N2 && need.spec_sub <= have.spec_sub && need.spec_full <= have.spec_full &&
+         need.fft <= have.fft && need.psf <= have.psf && need.ext <= have.ext && need.partial <= have.partial &&
+         need.pmax <= have.pmax;
+}
+
+static JobSizes max_sizes(const JobSizes& a, const JobSizes& b) {
+  JobSizes z;
+  z.s2 = std::max(a.s2, b.s2);
+  z.S_s2 = std::max(a.S_s2, b.S_s2);
+  z.mask = std::max(a.mask, b.mask);
+  z.iuwt = std::max(a.iuwt, b.iuwt);
+  z.N2 = std::max(a.N2, b.N2);
+  z.spec_sub = std::max(a.spec_sub, b.spec_sub);
+  z.spec_full = std::max(a.spec_full, b.spec_full);
+  z.fft = std::max(a.fft, b.fft);
+  z.psf = std::max(a.psf, b.psf);
+  z.ext = std::max(a.ext, b.ext);
+  z.partial = std::max(a.partial, b.partial);
+  z.pmax = std::max(a.pmax, b.pmax);
+  return z;
+}
+
+size_t job_footprint_bytes(const Geometry& g) {
+  const JobSizes z = job_sizes(g, std::max(g.S, (int)std::log2((double)g.n) - 1));
+  return z.s2 * 10 + z.S_s2 * 3 + z.mask + z.iuwt + z.N2 * 3 + z.spec_sub + z.spec_full + z.fft + z.psf + z.ext +
+         z.partial + z.pmax + 64 * 8 + 64 * 4 + 12 * 256;
+}
+
+// Buffers grow to the largest geometry seen and are reused by smaller jobs.
+void Job::ensure(const Geometry& g) {
+  const int S = std::max(g.S, (int)std::log2((double)g.n) - 1);
+  const JobSizes need = job_sizes(g, S);
+  geo_ = g;
+  if (have_ && fits(need, cap_)) return;
+  const JobSizes z = have_ ? max_sizes(need, cap_) : need;
+  arena_.release();
+  cap_ = z;
+  auto F = [&](size_t bytes) { return (float*)arena_.alloc(bytes); };
+  dirty_sub = F(z.s2);
+  psf_sub = F(z.s2);
+  D = F(z.S_s2);
+  Dcg = F(z.S_s2);
+  E = F(z.S_s2);
+  M = (uint8_t*)arena_.alloc(z.mask);
+  rec = F(z.s2);
+  x = F(z.s2);
+  xn = F(z.s2);
+  r = F(z.s2);
+  rn = F(z.s2);
+  p = F(z.s2);
+  ap = F(z.s2);
+  cbuf = F(z.s2);
+  iuwt_work = F(z.iuwt);
+  model_local = F(z.N2);
+  resid = F(z.N2);
+  conv_full = F(z.N2);
+  spec_sub = (float2*)arena_.alloc(z.spec_sub);
+  spec_full_buf = (float2*)arena_.alloc(z.spec_full);
+  fft_work = (float2*)arena_.alloc(z.fft);
+  psf_frame = F(z.psf);
+  mad_work = arena_.alloc(mad_work_bytes(16));
+  mad_work_init(mad_work, 16, st_);
+  ext_work = arena_.alloc(z.ext);
+  partial = (double*)arena_.alloc(z.partial);
+  pmax = (float*)arena_.alloc(z.pmax);
   scal = (double*)arena_.alloc(64 * sizeof(double));
   iflag = (int*)arena_.alloc(64 * sizeof(int));
   border_dev = iflag + 16;
@@ -242,6 +285,7 @@ void Job::ensure(const Geometry& g) {
 // psf_sub, PSF spectra and per-scale PSF energies (main.py:126-226)
 void Job::setup_psf(const float* psf_dev, std::vector<double>& energies) {
   const Geometry& g = geo_;
+  spec_full = (g.p_full == g.p_sub && g.sub == g.n) ? spec_sub : spec_full_buf;
   const int pn = g.psf_n;
   const int plo = g.double_psf ? pn / 2 - g.sub / 2 : g.lo;
   window_copy(psf_dev, pn, plo, plo, g.sub, nullptr, psf_sub, st_);

@@ -657,6 +657,9 @@ void source_extract(const float* w, int S, int h, int wd, double tol, int neg_co
   float* pmax = smax + 64;
   int* lab = (int*)((char*)work + ext_labels_offset(plane));
   const int gx = threshold_grid(plane);
+  // keep flags start cleared whatever the workspace held before
+  hipLaunchKernelGGL(k_clear_u8, dim3(ew_grid(plane)), dim3(kT), 0, st, keep, plane, pred);
+  PMS_LAUNCH_CHECK();
   plane_max(w, S, plane, neg_comp, pmax, pred, st);
   max_final(pmax, gx, S, smax, nullptr, nullptr, pred, st);
   const int gl = std::min(ew_grid(plane), 1024);

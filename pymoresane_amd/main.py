@@ -17,6 +17,7 @@ import logging
 import numpy as np
 
 from . import _lib
+from . import fits as _fits
 from ._lib import DeviceArray, MoresaneParams, TraceEvent, check
 from .iuwt import check_mode
 
@@ -160,3 +161,74 @@ class DataImage(object):
                           edge_suppression, edge_offset, flux_threshold, neg_comp, edge_excl, int_excl)
         self._run(True, start_scale, stop_scale, prm)
         self.complete = False
+
+    def restore(self):
+        """Model convolved with the fitted clean beam plus the residual (pymoresane/main.py:601-616).
+
+        The beam fit runs on the host (beam_fit); the convolution runs on the GPU:
+        linear (2N frame) when the PSF is twice the image, circular otherwise.
+        """
+        from .beam_fit import beam_fit
+        from .iuwt_convolution import fft_convolve, psf_spectrum
+        clean_beam, beam_params = beam_fit(self.psf_data, self.cdelt1, self.cdelt2)
+        n = self.dirty_data_shape[0]
+        mode = "linear" if self.psf_data_shape[0] == 2 * n else "circular"
+        conv = fft_convolve(self.model, psf_spectrum(clean_beam.astype(np.float32)), conv_mode=mode)
+        self.restored = (conv.astype(np.float64) + self.residual).astype(np.float32)
+        return beam_params
+
+    def handle_input(self, input_hdr):
+        """Index that keeps the RA/DEC axes of a FITS cube and takes plane 0 elsewhere (main.py:618-634)."""
+        naxis = int(input_hdr["NAXIS"])
+        idx = [0] * naxis
+        for i in range(naxis):
+            ctype = str(input_hdr.get("CTYPE%d" % (i + 1), ""))
+            if ctype.startswith("RA"):
+                idx[-1] = slice(None)
+            if ctype.startswith("DEC"):
+                idx[-2] = slice(None)
+        return tuple(idx)
+
+    def save_fits(self, data, name):
+        """Write data as a (1, 1, N, N) float32 FITS with the dirty image's header (main.py:636-646)."""
+        data = np.asarray(data)
+        _fits.write(name, data.reshape(1, 1, data.shape[0], data.shape[1]), getattr(self, "img_hdr", None))
+
+
+def _box5(a):
+    """5 x 5 box sum with zero padding, same size (mask smoothing, main.py:711-715)."""
+    p = np.pad(a, 2)
+    out = np.zeros_like(a, dtype=np.float64)
+    for dr in range(5):
+        for dc in range(5):
+            out += p[dr:dr + a.shape[0], dc:dc + a.shape[1]]
+    return out
+
+
+class FitsImage(DataImage):
+    """DataImage read from FITS files (pymoresane/main.py:678-733)."""
+
+    def __init__(self, image_name, psf_name, mask_name=None):
+        self.image_name = image_name
+        self.psf_name = psf_name
+        self.img_hdr, img = _fits.read(image_name)
+        self.psf_hdr, psf = _fits.read(psf_name)
+        cdelt1 = self.psf_hdr.get("CDELT1")
+        cdelt2 = self.psf_hdr.get("CDELT2", cdelt1)
+        mask = None
+        if mask_name is not None:
+            _, m = _fits.read(mask_name)
+            m = m.reshape(m.shape[-2], m.shape[-1]).astype(np.float64)
+            m = m / np.max(m)
+            m = _box5(m)
+            mask = m / np.max(m)
+        super(FitsImage, self).__init__(img[self.handle_input(self.img_hdr)], psf[self.handle_input(self.psf_hdr)],
+                                        mask, cdelt1, cdelt2)
+        self.mask_name = mask_name
+
+    def restore(self):
+        beam_params = super(FitsImage, self).restore()
+        self.img_hdr.set("BMAJ", float(beam_params[0]))
+        self.img_hdr.set("BMIN", float(beam_params[1]))
+        self.img_hdr.set("BPA", float(beam_params[2]))
+        return beam_params

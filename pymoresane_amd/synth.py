@@ -65,13 +65,21 @@ def make_sky(n, rng, n_points=40, flux_lo=-3.0, flux_hi=0.0, n_blobs=4, sig_lo=2
 
 
 def linear_convolve(sky, psf):
-    """``out[i, j] = sum_uv sky[u, v] psf[i-u+N, j-v+N]`` for an N x N sky and 2N x 2N psf (float64)."""
+    """``out[i, j] = sum_uv sky[u, v] psf[i-u+N, j-v+N]`` for an N x N sky (float64).
+
+    A circular convolution of size max(P, 2N) has no wrap-around on output
+    rows/columns [N, 2N), so that window is the linear result.
+    """
+    from scipy import fft as sfft
     n = sky.shape[0]
     p = psf.shape[0]
-    size = 2 * p
-    spec = np.fft.rfft2(sky.astype(np.float64), s=(size, size)) * \
-        np.fft.rfft2(psf.astype(np.float64), s=(size, size))
-    full = np.fft.irfft2(spec, s=(size, size))
+    size = max(p, 2 * n)
+    a = np.zeros((size, size))
+    a[:n, :n] = sky
+    b = np.zeros((size, size))
+    b[:p, :p] = psf
+    spec = sfft.rfft2(a, workers=-1) * sfft.rfft2(b, workers=-1)
+    full = sfft.irfft2(spec, s=(size, size), workers=-1)
     return full[n:2 * n, n:2 * n]
 
 
