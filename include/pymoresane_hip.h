@@ -196,6 +196,7 @@ typedef struct pms_batch_job {
   int complete;     /* out */
   double seconds;   /* out: wall time of the job */
   double t_start;   /* out: start offset from the batch start, seconds */
+  double wait_seconds; /* out: host time spent waiting for the job's stream */
 } pms_batch_job;
 
 int pms_batch_run(pms_batch_job* jobs, int n_jobs, int n_workers);

@@ -45,7 +45,7 @@ class BatchJob(ctypes.Structure):
     _fields_ = [("dirty", c_void_p), ("psf", c_void_p), ("mask", c_void_p), ("model", c_void_p),
                 ("residual", c_void_p), ("prm", MoresaneParams), ("by_scale", c_int), ("start_scale", c_int),
                 ("stop_scale", c_int), ("status", c_int), ("major_iters", c_int), ("complete", c_int),
-                ("seconds", c_double), ("t_start", c_double)]
+                ("seconds", c_double), ("t_start", c_double), ("wait_seconds", c_double)]
 
 
 class ServeRequest(ctypes.Structure):

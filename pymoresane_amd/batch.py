@@ -75,7 +75,7 @@ def run_batch(jobs, n_workers=8, order=None, by_scale=False, start_scale=1, stop
         b = arr[slot]
         if b.status != 0:
             raise _lib.PmsError(b.status, "batch job %d failed" % i)
-        out[i] = (b.status, b.major_iters, b.seconds, b.t_start)
+        out[i] = (b.status, b.major_iters, b.seconds, b.t_start, b.wait_seconds)
     return out
 
 

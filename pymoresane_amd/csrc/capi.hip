@@ -390,6 +390,7 @@ int pms_batch_run(pms_batch_job* jobs, int n_jobs, int n_workers) {
             const auto a = std::chrono::steady_clock::now();
             J.t_start = std::chrono::duration<double>(a - t0).count();
             Trace tr;
+            job.reset_wait();
             J.status = guarded([&] {
               check_params(&J.prm);
               if (J.by_scale) {
@@ -404,6 +405,7 @@ int pms_batch_run(pms_batch_job* jobs, int n_jobs, int n_workers) {
               }
             });
             J.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+            J.wait_seconds = job.wait_seconds();
           }
         }
         hipStreamDestroy(st);

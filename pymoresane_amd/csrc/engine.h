@@ -1,6 +1,8 @@
 // Device-resident MORESANE driver: per-job state and the major/minor loops.
 #pragma once
 
+#include <map>
+#include <tuple>
 #include <vector>
 
 #include "../../include/pymoresane_hip.h"
@@ -53,6 +55,17 @@ struct JobSizes {
 // device bytes one job of this geometry needs (its workspace plus nothing else)
 size_t job_footprint_bytes(const Geometry& g);
 
+// Buffers and scales of one CG iteration: the key of a captured iteration graph.
+struct IterArgs {
+  float *x, *xn, *r, *rn, *p, *ap;
+  double *rr, *rnrn;
+  int max_scale, adj, enforce, sub, p_sub;
+  bool operator<(const IterArgs& o) const {
+    return std::tie(x, xn, r, rn, p, ap, rr, rnrn, max_scale, adj, enforce, sub, p_sub) <
+           std::tie(o.x, o.xn, o.r, o.rn, o.p, o.ap, o.rr, o.rnrn, o.max_scale, o.adj, o.enforce, o.sub, o.p_sub);
+  }
+};
+
 // Device buffers needed by one moresane() call, kept across calls of the same shape.
 class Job {
  public:
@@ -64,6 +77,8 @@ class Job {
                          const float* psf_dev, const float* mask_dev, float* model_dev, float* residual_dev,
                          Trace& trace);
   size_t device_bytes() const { return arena_.bytes(); }
+  double wait_seconds() const { return wait_seconds_; }
+  void reset_wait() { wait_seconds_ = 0.0; }
   hipStream_t stream() const { return st_; }
 
  private:
@@ -71,8 +86,15 @@ class Job {
   void setup_psf(const float* psf_dev, std::vector<double>& energies);
   void apply_op_masked(const float* v, float* out, int max_scale, int adj, const int* pred = nullptr);
   double residual_and_std(const float* dirty_dev, const float* model_dev);
+  void enqueue_iteration(const IterArgs& a);
+  void run_iteration(const IterArgs& a, bool use_graph);
+  void drop_graphs();
+  void sync();
+  double wait_seconds_ = 0.0;
+  std::map<IterArgs, hipGraphExec_t> graphs_;
 
   hipStream_t st_;
+  bool owns_stream_ = false;
   DeviceArena arena_;
   DeviceArena by_scale_arena_;
   Geometry geo_{};

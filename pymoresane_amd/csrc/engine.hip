@@ -17,6 +17,7 @@
 // Kernels run with a null predicate here; the predicate mechanism exists so a
 // caller can enqueue work whose execution depends on device state.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <vector>
 
@@ -27,32 +28,8 @@ namespace {
 
 constexpr int kT = 256;
 
-__global__ void k_div(const double* a, const double* b, double* out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *out = *a / *b;
-}
 
-// xn = x + alpha p ; partial min of xn
-__global__ __launch_bounds__(kT) void k_cg_xn(const float* __restrict__ x, const float* __restrict__ p,
-                                              const double* __restrict__ alpha, float* __restrict__ xn, size_t n,
-                                              float* __restrict__ pmin) {
-  const double a = *alpha;
-  float m = INFINITY;
-  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) {
-    const float v = (float)fma(a, (double)p[i], (double)x[i]);
-    xn[i] = v;
-    m = fminf(m, v);
-  }
-  m = block_min_f32(m);
-  if (threadIdx.x == 0) pmin[blockIdx.x] = m;
-}
 
-__global__ void k_cg_clipflag(const float* pmin, int nb, int enforce, int* flag, float* xmin) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  float m = INFINITY;
-  for (int b = 0; b < nb; ++b) m = fminf(m, pmin[b]);
-  *xmin = m;
-  *flag = (enforce && m < 0.f) ? 1 : 0;
-}
 
 // positivity clip: xn[xn<0] = 0 ; p = (xn - x) / alpha
 __global__ __launch_bounds__(kT) void k_cg_clip(float* __restrict__ xn, const float* __restrict__ x,
@@ -68,20 +45,6 @@ __global__ __launch_bounds__(kT) void k_cg_clip(float* __restrict__ xn, const fl
   }
 }
 
-// rn = r - alpha Ap ; partial sum rn^2
-__global__ __launch_bounds__(kT) void k_cg_rn(const float* __restrict__ r, const float* __restrict__ ap,
-                                              const double* __restrict__ alpha, float* __restrict__ rn, size_t n,
-                                              double* __restrict__ partial) {
-  const double a = *alpha;
-  double acc = 0.0;
-  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) {
-    const float v = (float)fma(-a, (double)ap[i], (double)r[i]);
-    rn[i] = v;
-    acc = fma((double)v, (double)v, acc);
-  }
-  acc = block_sum_f64(acc);
-  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
-}
 
 // p = rn + beta p
 __global__ __launch_bounds__(kT) void k_cg_p(const float* __restrict__ rn, float* __restrict__ p,
@@ -111,6 +74,91 @@ __global__ __launch_bounds__(kT) void k_sub(const float* __restrict__ a, const f
 // plane += other
 __global__ __launch_bounds__(kT) void k_add_inplace(float* __restrict__ a, const float* __restrict__ b, size_t n) {
   for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) a[i] += b[i];
+}
+
+// Deterministic grid-wide finish: every block publishes its partial, the last block to arrive sums
+// the partials in block order (fixed tree) and runs `fin` on the total; the counter is left at 0.
+template <class Fin>
+__device__ __forceinline__ void finish_sum(double part, double* partial, unsigned* counter, Fin fin) {
+  __shared__ int is_last;
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = part;
+    __threadfence();
+    is_last = (atomicAdd(counter, 1u) == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  double acc = 0.0;
+  for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x) acc += ((volatile const double*)partial)[i];
+  acc = block_sum_f64(acc);
+  if (threadIdx.x == 0) {
+    fin(acc);
+    *counter = 0u;
+  }
+}
+
+// alpha = rr / (p . Ap)
+__global__ __launch_bounds__(kT) void k_cg_alpha(const float* __restrict__ p, const float* __restrict__ ap, size_t n,
+                                                 double* partial, unsigned* counter, const double* rr, double* pap,
+                                                 double* alpha) {
+  double acc = 0.0;
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT)
+    acc = fma((double)p[i], (double)ap[i], acc);
+  acc = block_sum_f64(acc);
+  finish_sum(acc, partial, counter, [&](double tot) {
+    *pap = tot;
+    *alpha = *rr / tot;
+  });
+}
+
+// xn = x + alpha p ; clip flag = enforce && min(xn) < 0
+__global__ __launch_bounds__(kT) void k_cg_xn_flag(const float* __restrict__ x, const float* __restrict__ p,
+                                                   const double* __restrict__ alpha, float* __restrict__ xn, size_t n,
+                                                   double* partial, unsigned* counter, int enforce, int* flag) {
+  const double a = *alpha;
+  float m = INFINITY;
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) {
+    const float v = (float)fma(a, (double)p[i], (double)x[i]);
+    xn[i] = v;
+    m = fminf(m, v);
+  }
+  m = block_min_f32(m);
+  // min via the same last-block pattern: publish min as a double, reduce with min in block order
+  __shared__ int is_last;
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = (double)m;
+    __threadfence();
+    is_last = (atomicAdd(counter, 1u) == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  if (threadIdx.x == 0) {
+    double mn = INFINITY;
+    for (unsigned i = 0; i < gridDim.x; ++i) mn = fmin(mn, ((volatile const double*)partial)[i]);
+    *flag = (enforce && mn < 0.0) ? 1 : 0;
+    *counter = 0u;
+  }
+}
+
+// rn = r - alpha Ap ; rnrn = rn . rn ; beta = rnrn / rr
+__global__ __launch_bounds__(kT) void k_cg_rn_beta(const float* __restrict__ r, const float* __restrict__ ap,
+                                                   const double* __restrict__ alpha, float* __restrict__ rn, size_t n,
+                                                   double* partial, unsigned* counter, const double* rr, double* rnrn,
+                                                   double* beta) {
+  const double a = *alpha;
+  double acc = 0.0;
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) {
+    const float v = (float)fma(-a, (double)ap[i], (double)r[i]);
+    rn[i] = v;
+    acc = fma((double)v, (double)v, acc);
+  }
+  acc = block_sum_f64(acc);
+  finish_sum(acc, partial, counter, [&](double tot) {
+    *rnrn = tot;
+    *beta = tot / *rr;
+  });
 }
 
 int grid_for(size_t n) { return reduce_grid(n); }
@@ -182,8 +230,27 @@ Geometry make_geometry(int n, int psf_n, const MoresaneParams& prm, int scale_co
 }
 
 // --------------------------------------------------------------------------- job
-Job::Job(hipStream_t st) : st_(st) {}
-Job::~Job() {}
+// Host waits on the job's stream, timed so callers can tell GPU-bound from host-bound runs.
+void Job::sync() {
+  const auto t0 = std::chrono::steady_clock::now();
+  PMS_HIP(hipStreamSynchronize(st_));
+  wait_seconds_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// A job handed the legacy NULL stream runs on a stream of its own (graph capture needs one); every
+// moresane()/moresane_by_scale() call synchronises that stream before returning.
+Job::Job(hipStream_t st) : st_(st) {
+  if (!st_) {
+    PMS_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    owns_stream_ = true;
+  }
+}
+Job::~Job() {
+  drop_graphs();
+  arena_.release();
+  by_scale_arena_.release();
+  if (owns_stream_) (void)hipStreamDestroy(st_);
+}
 
 // Byte sizes of every per-job buffer for geometry g (S = scale capacity).
 static JobSizes job_sizes(const Geometry& g, int S) {
@@ -246,6 +313,7 @@ void Job::ensure(const Geometry& g) {
   geo_ = g;
   if (have_ && fits(need, cap_)) return;
   const JobSizes z = have_ ? max_sizes(need, cap_) : need;
+  drop_graphs();
   arena_.release();
   cap_ = z;
   auto F = [&](size_t bytes) { return (float*)arena_.alloc(bytes); };
@@ -278,6 +346,7 @@ void Job::ensure(const Geometry& g) {
   pmax = (float*)arena_.alloc(z.pmax);
   scal = (double*)arena_.alloc(64 * sizeof(double));
   iflag = (int*)arena_.alloc(64 * sizeof(int));
+  fill_u32((uint32_t*)iflag, 64, 0u, st_);  // flags and the grid-finish counters start at zero
   border_dev = iflag + 16;
   have_ = true;
 }
@@ -333,7 +402,7 @@ void Job::setup_psf(const float* psf_dev, std::vector<double>& energies) {
   sum_partials(partial, gr, g.S, scal + 32, nullptr, st_);
   std::vector<double> e2(g.S);
   PMS_HIP(hipMemcpyAsync(e2.data(), scal + 32, sizeof(double) * g.S, hipMemcpyDeviceToHost, st_));
-  PMS_HIP(hipStreamSynchronize(st_));
+  sync();
   energies.resize(g.S);
   for (int i = 0; i < g.S; ++i) energies[i] = std::sqrt(e2[i]);
   PMS_HIP(hipMemcpyAsync(scal + 32, energies.data(), sizeof(double) * g.S, hipMemcpyHostToDevice, st_));
@@ -351,7 +420,7 @@ double Job::residual_and_std(const float* dirty_dev, const float* model_dev) {
   mean_std(dirty_sub, s2, partial, scal + 56, scal + 57, st_);
   double v[2];
   PMS_HIP(hipMemcpyAsync(v, scal + 56, sizeof(double) * 2, hipMemcpyDeviceToHost, st_));
-  PMS_HIP(hipStreamSynchronize(st_));
+  sync();
   return std::sqrt(v[1] / (double)s2);
 }
 
@@ -393,22 +462,21 @@ int Job::moresane(const MoresaneParams& prm, int scale_count, const float* dirty
   max_final(pmax, threshold_grid(s2), 1, (float*)(scal + 60), nullptr, nullptr, nullptr, st_);
   float dmax = 0.f;
   PMS_HIP(hipMemcpyAsync(&dmax, scal + 60, sizeof(float), hipMemcpyDeviceToHost, st_));
-  PMS_HIP(hipStreamSynchronize(st_));
+  sync();
 
   double* thr = scal + 0;       // [0, 16)  thresholds (S <= 16)
   double* nsm_dev = scal + 16;  // [16, 32) normalised maxima
   double* energy_dev = scal + 32;
   double* d_rr = scal + 48;
-  double* d_pap = scal + 49;
-  double* d_alpha = scal + 50;
   double* d_rnrn = scal + 51;
-  double* d_beta = scal + 52;
   double* d_d2 = scal + 53;
   double* d_e2 = scal + 54;
-  int* clip = iflag + 0;
-  float* xmin = (float*)(iflag + 1);
   PMS_REQUIRE(S <= 16, "scale_count above 16 is not supported");
 
+  // Replaying the iteration from a captured HIP graph is bit-identical to eager launches but measured
+  // slower on ROCm 7 for large frames, so it is opt-in (PMS_CG_GRAPH=1).
+  const char* graph_env = getenv("PMS_CG_GRAPH");
+  const bool use_graph = graph_env && graph_env[0] == '1';
   int major = 0;
   double max_coeff = 1.0;
   double std_current = 1000.0, std_ratio = 1.0;
@@ -433,7 +501,7 @@ int Job::moresane(const MoresaneParams& prm, int scale_count, const float* dirty
                         nullptr, st_);
         max_final(pmax, pmx, S, nullptr, energy_dev, nsm_dev, nullptr, st_);
         PMS_HIP(hipMemcpyAsync(nsm.data(), nsm_dev, sizeof(double) * S, hipMemcpyDeviceToHost, st_));
-        PMS_HIP(hipStreamSynchronize(st_));
+        sync();
       }
       int max_index = min_scale;
       for (int i = min_scale + 1; i < S; ++i)
@@ -474,39 +542,12 @@ int Job::moresane(const MoresaneParams& prm, int scale_count, const float* dirty
       double snr_last = 0.0, snr_current = 0.0;
       int exit_code = EXIT_CAP;
       while (niter < prm.minor_loop_miter) {
-        apply_op_masked(p, ap, max_scale, scale_adjust);
-        dot_partials(p, ap, s2, partial, nullptr, st_);
-        sum_partials(partial, gr, 1, d_pap, nullptr, st_);
-        hipLaunchKernelGGL(k_div, dim3(1), dim3(1), 0, st_, (const double*)d_rr, (const double*)d_pap, d_alpha);
-        PMS_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_cg_xn, dim3(gr), dim3(kT), 0, st_, (const float*)x, (const float*)p,
-                           (const double*)d_alpha, xn, s2, pmax);
-        PMS_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_cg_clipflag, dim3(1), dim3(1), 0, st_, (const float*)pmax, gr, prm.enforce_positivity,
-                           clip, xmin);
-        PMS_LAUNCH_CHECK();
-        if (prm.enforce_positivity) {
-          hipLaunchKernelGGL(k_cg_clip, dim3(gr), dim3(kT), 0, st_, xn, (const float*)x, p, (const double*)d_alpha,
-                             s2, (const int*)clip);
-          PMS_LAUNCH_CHECK();
-          apply_op_masked(p, ap, max_scale, scale_adjust, clip);
-        }
-        hipLaunchKernelGGL(k_cg_rn, dim3(gr), dim3(kT), 0, st_, (const float*)r, (const float*)ap,
-                           (const double*)d_alpha, rn, s2, partial);
-        PMS_LAUNCH_CHECK();
-        sum_partials(partial, gr, 1, d_rnrn, nullptr, st_);
-        hipLaunchKernelGGL(k_div, dim3(1), dim3(1), 0, st_, (const double*)d_rnrn, (const double*)d_rr, d_beta);
-        PMS_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_cg_p, dim3(gr), dim3(kT), 0, st_, (const float*)rn, p, (const double*)d_beta, s2);
-        PMS_LAUNCH_CHECK();
-        // model sources and the SNR sums (main.py:413-424)
-        fft_conv(xn, g.sub, g.p_sub, spec_sub, cbuf, fft_work, nullptr, st_);
-        const long np = iuwt_decompose(cbuf, g.sub, g.sub, max_scale, scale_adjust, nullptr, nullptr, iuwt_work,
-                                       Ms, Es, partial, nullptr, st_);
-        sum_partials(partial, np, 1, d_d2, nullptr, st_);
+        IterArgs ia{x, xn, r, rn, p, ap, d_rr, d_rnrn, max_scale, scale_adjust, prm.enforce_positivity, g.sub,
+                    g.p_sub};
+        run_iteration(ia, use_graph);
         double d2 = 0.0;
         PMS_HIP(hipMemcpyAsync(&d2, d_d2, sizeof(double), hipMemcpyDeviceToHost, st_));
-        PMS_HIP(hipStreamSynchronize(st_));
+        sync();
         snr_last = snr_current;
         snr_current = 20.0 * std::log10(std::sqrt(e2) / std::sqrt(d2));
         ++niter;
@@ -568,7 +609,7 @@ int Job::moresane(const MoresaneParams& prm, int scale_count, const float* dirty
       plane_max(dirty_sub, 1, s2, 0, pmax, nullptr, st_);
       max_final(pmax, pmx, 1, (float*)(scal + 60), nullptr, nullptr, nullptr, st_);
       PMS_HIP(hipMemcpyAsync(&dmax, scal + 60, sizeof(float), hipMemcpyDeviceToHost, st_));
-      PMS_HIP(hipStreamSynchronize(st_));
+      sync();
       ++major;
     }
     if (major == 0) {
@@ -582,8 +623,80 @@ int Job::moresane(const MoresaneParams& prm, int scale_count, const float* dirty
     PMS_LAUNCH_CHECK();
     copy_plane(resid, residual_dev, N2, nullptr, st_);
   }
-  PMS_HIP(hipStreamSynchronize(st_));
+  sync();
   return major;
+}
+
+// One CG iteration (main.py:380-424) up to the SNR sums, enqueued on the job's stream.
+void Job::enqueue_iteration(const IterArgs& a) {
+  const Geometry& g = geo_;
+  const size_t s2 = (size_t)g.sub * g.sub;
+  const int gr = reduce_grid(s2);
+  double* d_pap = scal + 49;
+  double* d_alpha = scal + 50;
+  double* d_beta = scal + 52;
+  double* d_d2 = scal + 53;
+  int* clip = iflag + 0;
+  unsigned* ctr = (unsigned*)(iflag + 40);
+  uint8_t* Ms = M + (size_t)a.adj * s2;
+  const float* Es = E + (size_t)a.adj * s2;
+  apply_op_masked(a.p, a.ap, a.max_scale, a.adj);
+  hipLaunchKernelGGL(k_cg_alpha, dim3(gr), dim3(kT), 0, st_, (const float*)a.p, (const float*)a.ap, s2, partial,
+                     ctr, (const double*)a.rr, d_pap, d_alpha);
+  PMS_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_cg_xn_flag, dim3(gr), dim3(kT), 0, st_, (const float*)a.x, (const float*)a.p,
+                     (const double*)d_alpha, a.xn, s2, partial, ctr + 1, a.enforce, clip);
+  PMS_LAUNCH_CHECK();
+  if (a.enforce) {
+    hipLaunchKernelGGL(k_cg_clip, dim3(gr), dim3(kT), 0, st_, a.xn, (const float*)a.x, a.p, (const double*)d_alpha,
+                       s2, (const int*)clip);
+    PMS_LAUNCH_CHECK();
+    apply_op_masked(a.p, a.ap, a.max_scale, a.adj, clip);
+  }
+  hipLaunchKernelGGL(k_cg_rn_beta, dim3(gr), dim3(kT), 0, st_, (const float*)a.r, (const float*)a.ap,
+                     (const double*)d_alpha, a.rn, s2, partial, ctr + 2, (const double*)a.rr, a.rnrn, d_beta);
+  PMS_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_cg_p, dim3(gr), dim3(kT), 0, st_, (const float*)a.rn, a.p, (const double*)d_beta, s2);
+  PMS_LAUNCH_CHECK();
+  // model sources and the SNR sums (main.py:413-424)
+  fft_conv(a.xn, g.sub, g.p_sub, spec_sub, cbuf, fft_work, nullptr, st_);
+  const long np = iuwt_decompose(cbuf, g.sub, g.sub, a.max_scale, a.adj, nullptr, nullptr, iuwt_work, Ms, Es,
+                                 partial, nullptr, st_);
+  sum_partials(partial, np, 1, d_d2, nullptr, st_);
+}
+
+// Run one iteration, replaying a captured HIP graph for this exact set of buffers and scales when
+// enabled (the first iteration of each configuration records it).
+void Job::run_iteration(const IterArgs& a, bool use_graph) {
+  if (!use_graph) {
+    enqueue_iteration(a);
+    return;
+  }
+  auto it = graphs_.find(a);
+  if (it == graphs_.end()) {
+    PMS_HIP(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue_iteration(a);
+    } catch (...) {
+      hipGraph_t junk = nullptr;
+      (void)hipStreamEndCapture(st_, &junk);
+      if (junk) (void)hipGraphDestroy(junk);
+      throw;
+    }
+    hipGraph_t graph = nullptr;
+    PMS_HIP(hipStreamEndCapture(st_, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    PMS_HIP(e);
+    it = graphs_.emplace(a, exec).first;
+  }
+  PMS_HIP(hipGraphLaunch(it->second, st_));
+}
+
+void Job::drop_graphs() {
+  for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+  graphs_.clear();
 }
 
 // Ap <- R(M . W(H v)) over scales [adj, max_scale); masks are indexed by scale
@@ -614,7 +727,7 @@ void Job::moresane_by_scale(const MoresaneParams& prm, int start_scale, int stop
     if (scale_count > std::log2((double)n) - 1) break;
     if (scale_count > stop_scale) break;
   }
-  PMS_HIP(hipStreamSynchronize(st_));
+  sync();
   by_scale_arena_.release();
 }
 

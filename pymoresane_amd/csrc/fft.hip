@@ -451,18 +451,23 @@ int env_int(const char* name, int dflt) {
   return v ? atoi(v) : dflt;
 }
 
-int rows_per_block(int L) {
+// Rows / columns per workgroup: as many as the LDS budget allows for wide coalesced row accesses,
+// but never so many that the launch has fewer than ~256 workgroups (small frames stay parallel).
+int rows_per_block(int L, int n_rows) {
   static const int lim = env_int("PMS_FFT_ROW_LDS", 8192);
   int rb = lim / (L > 0 ? L : 1);
-  if (rb < 1) rb = 1;
   if (rb > 16) rb = 16;
+  while (rb > 1 && n_rows / rb < 256) rb >>= 1;
+  if (rb < 1) rb = 1;
   return rb;
 }
 
 int cols_per_block(int P) {
+  const int ncols = P / 2 + 1;
   int c = 8192 / P;
-  if (c < 1) c = 1;
   if (c > 16) c = 16;
+  while (c > 1 && ncols / c < 256) c >>= 1;
+  if (c < 1) c = 1;
   return c;
 }
 
@@ -488,7 +493,7 @@ int first_radix(int m) { return 1 << plan_first_log2(m); }
 void launch_rows_r2c(const float* src, int q, int off, int pm, int n_rows, float2* T, int ldT, const float2* tw,
                      const int* pred, hipStream_t st) {
   const int L = (1 << pm) / 2;
-  const int rb = rows_per_block(L);
+  const int rb = rows_per_block(L, n_rows);
   const size_t lds = (size_t)rb * (L + (L >> 5)) * sizeof(float2);
   PMS_FFT_DISPATCH(first_radix(pm - 1), k_rows_r2c, dim3(ceil_div(n_rows, rb)), dim3(kThreads), lds, st, src, q, off,
                    pm, n_rows, rb, T, ldT, tw, pred);
@@ -498,7 +503,7 @@ void launch_rows_r2c(const float* src, int q, int off, int pm, int n_rows, float
 void launch_rows_c2r(const float2* T2, int ldT, int pm, int n_rows, float* out, int n_cols, float scale,
                      const float2* tw, const int* pred, hipStream_t st) {
   const int L = (1 << pm) / 2;
-  const int rb = rows_per_block(L);
+  const int rb = rows_per_block(L, n_rows);
   const size_t lds = (size_t)rb * (L + (L >> 5)) * sizeof(float2);
   PMS_FFT_DISPATCH(first_radix(pm - 1), k_rows_c2r, dim3(ceil_div(n_rows, rb)), dim3(kThreads), lds, st, T2, ldT,
                    pm, n_rows, rb, out, n_cols, scale, tw, pred);

@@ -224,6 +224,75 @@ __global__ __launch_bounds__(kThreads) void k_iuwt_scale(ScaleParams a) {
   }
 }
 
+// Small planes (whole plane L2-resident): one thread per output pixel loads its full dilated
+// 9x9 (decomposition) or 5x5 neighbourhood at once, so a scale costs one memory latency instead
+// of a row walk.  Same arithmetic as k_iuwt_scale: row sums first, then the vertical combination.
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void k_iuwt_small(ScaleParams a) {
+  constexpr int H = ModeTraits<MODE>::H;
+  constexpr int T = 2 * H + 1;
+  if (a.pred && *a.pred == 0) return;
+  const int w = a.w, h = a.h, d = a.d;
+  const int col0 = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int row0 = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const bool live = col0 < w && row0 < h;
+  const int col = live ? col0 : 0;
+  const int row = live ? row0 : 0;
+  int cidx[T];
+#pragma unroll
+  for (int b = 0; b < T; ++b) cidx[b] = reflect_idx(col + (b - H) * d, w);
+  float ra[T], rb[T], xc = 0.f;
+#pragma unroll
+  for (int i = 0; i < T; ++i) {
+    const float* src = a.in + (size_t)reflect_idx(row + (i - H) * d, h) * w;
+    float v[T];
+#pragma unroll
+    for (int b = 0; b < T; ++b) v[b] = src[cidx[b]];
+    float sa = 0.f, sb = 0.f;
+#pragma unroll
+    for (int b = 0; b < 5; ++b) sa = fmaf(c_h5[b], v[b + H - 2], sa);
+    if (MODE == IUWT_DECOMP) {
+#pragma unroll
+      for (int b = 0; b < 9; ++b) sb = fmaf(c_h9[b], v[b], sb);
+      if (i == H) xc = v[H];
+    }
+    ra[i] = sa;
+    rb[i] = sb;
+  }
+  float c = 0.f;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) c = fmaf(c_h5[k], ra[k + H - 2], c);
+  const size_t off = (size_t)row * w + col;
+  if (MODE == IUWT_DECOMP) {
+    float c1 = 0.f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) c1 = fmaf(c_h9[k], rb[k], c1);
+    float wv = xc - c1;
+    if (live) {
+      if (a.mask && a.mask[off] == 0) wv = 0.f;
+      if (a.out_c) a.out_c[off] = c;
+      if (a.out_w) a.out_w[off] = wv;
+    }
+    if (a.ref) {
+      double acc = 0.0;
+      if (live) {
+        const float df = a.ref[off] - wv;
+        acc = (double)df * (double)df;
+      }
+      acc = block_sum_f64(acc);
+      if (threadIdx.x == 0) a.partial[blockIdx.y * gridDim.x + blockIdx.x] = acc;
+    }
+  } else if (!live) {
+    return;
+  } else if (MODE == IUWT_SMOOTH) {
+    a.out_c[off] = c;
+  } else {
+    float wi = a.w_in[off];
+    if (a.mask && a.mask[off] == 0) wi = 0.f;
+    a.out_c[off] = c + wi;
+  }
+}
+
 struct Plan {
   int q_log2, tj, k_rows, n_cg, n_tb, n_seg, blocks, lds_bytes;
 };
@@ -236,11 +305,13 @@ int env_int(const char* name, int dflt) {
 // tuning knobs, read once per process
 struct IuwtTuning {
   int qmax, krows, pf, target_blocks;
+  long small_pixels;  // planes up to this many pixels use the direct-stencil kernel
   IuwtTuning()
       : qmax(env_int("PMS_IUWT_QMAX", 16)),
         krows(env_int("PMS_IUWT_KROWS", 0)),
         pf(env_int("PMS_IUWT_PF", 3)),
-        target_blocks(env_int("PMS_IUWT_BLOCKS", 2048)) {}
+        target_blocks(env_int("PMS_IUWT_BLOCKS", 2048)),
+        small_pixels(env_int("PMS_IUWT_SMALL", 512 * 512)) {}
 };
 const IuwtTuning& tuning() {
   static const IuwtTuning t;
@@ -320,8 +391,35 @@ void check_iuwt_scale(int h, int w, int scale) {
                   std::to_string(h) + "x" + std::to_string(w) + ")");
 }
 
+bool use_small(int h, int w) { return (long)h * w <= tuning().small_pixels; }
+
+dim3 small_grid(int h, int w) { return dim3(ceil_div(w, 64), ceil_div(h, 4)); }
+
 int iuwt_scale(const IuwtScaleCall& c, hipStream_t st) {
   check_iuwt_scale(c.h, c.w, c.scale);
+  if (use_small(c.h, c.w)) {
+    ScaleParams sp{};
+    sp.in = c.in;
+    sp.out_c = c.out_c;
+    sp.out_w = c.out_w;
+    sp.w_in = c.w_in;
+    sp.mask = c.mask;
+    sp.ref = c.ref;
+    sp.partial = c.partial;
+    sp.pred = c.pred;
+    sp.h = c.h;
+    sp.w = c.w;
+    sp.d = 1 << c.scale;
+    const dim3 grid = small_grid(c.h, c.w);
+    if (c.mode == IUWT_DECOMP)
+      hipLaunchKernelGGL(k_iuwt_small<IUWT_DECOMP>, grid, dim3(kThreads), 0, st, sp);
+    else if (c.mode == IUWT_SMOOTH)
+      hipLaunchKernelGGL(k_iuwt_small<IUWT_SMOOTH>, grid, dim3(kThreads), 0, st, sp);
+    else
+      hipLaunchKernelGGL(k_iuwt_small<IUWT_RECOMP>, grid, dim3(kThreads), 0, st, sp);
+    PMS_LAUNCH_CHECK();
+    return (int)(grid.x * grid.y);
+  }
   const int halo = (c.mode == IUWT_DECOMP) ? 4 : 2;
   Plan pl = make_plan(c.h, c.w, c.scale, halo);
   PMS_REQUIRE((pl.tj + 2 * halo) * (1 << pl.q_log2) <= 8 * kThreads, "IUWT tile too wide");
@@ -353,6 +451,10 @@ int iuwt_scale(const IuwtScaleCall& c, hipStream_t st) {
 }
 
 int iuwt_scale_blocks(int h, int w, int scale, int mode) {
+  if (use_small(h, w)) {
+    const dim3 g = small_grid(h, w);
+    return (int)(g.x * g.y);
+  }
   return make_plan(h, w, scale, mode == IUWT_DECOMP ? 4 : 2).blocks;
 }
 
