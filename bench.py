@@ -68,8 +68,10 @@ def pct(vals, q):
     return float(np.percentile(np.asarray(vals), q)) if len(vals) else None
 
 
-def bench_batched(workers=16):
-    """96 ragged pairs (32x256^2, 32x512^2, 24x1024^2, 8x2048^2), S=5: plain (one worker) vs batched."""
+def bench_batched(max_batch=32, workers=4):
+    """96 ragged pairs (32x256^2, 32x512^2, 24x1024^2, 8x2048^2), S=5: plain (one job at a time) vs
+    batched (each size class stepped in lockstep, up to max_batch jobs per batched launch, classes
+    concurrent on `workers` streams)."""
     import numpy as np
     from pymoresane_amd import batch, synth
     sides = synth.config_c3_sides()
@@ -81,12 +83,12 @@ def bench_batched(workers=16):
     order = sorted(range(len(sides)), key=lambda i: -sides[i])  # largest first
     # plain: one job at a time on one stream
     t0 = time.perf_counter()
-    plain_stats = batch.run_batch(jobs, n_workers=1, order=order)
+    plain_stats = batch.run_batch(jobs, n_workers=1, max_batch=1, order=order)
     t_plain = time.perf_counter() - t0
     ref = jobs.results()
     jobs.reset()
     t0 = time.perf_counter()
-    stats = batch.run_batch(jobs, n_workers=workers, order=order)
+    stats = batch.run_batch(jobs, n_workers=workers, max_batch=max_batch, order=order)
     t_batch = time.perf_counter() - t0
     same = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) for a, b in zip(jobs.results(), ref))
     lat = [s[2] for s in stats]
@@ -94,11 +96,12 @@ def bench_batched(workers=16):
     return {"workload": "c3: 96 ragged pairs 32x256^2+32x512^2+24x1024^2+8x2048^2, PSF 2N, S=5",
             "plain_images_per_s": round(len(sides) / t_plain, 3),
             "batched_images_per_s": round(len(sides) / t_batch, 3),
-            "speedup": round(t_plain / t_batch, 3), "workers": workers, "identical_to_alone": bool(same),
+            "speedup": round(t_plain / t_batch, 3), "max_batch": max_batch, "class_streams": workers,
+            "identical_to_alone": bool(same),
             "batched_job_p50_s": round(pct(lat, 50), 4), "batched_job_p99_s": round(pct(lat, 99), 4)}
 
 
-def bench_served(count=60, workers=16, rate=20.0, budget=200e9):
+def bench_served(count=60, max_batch=32, rate=20.0, budget=200e9):
     """Poisson arrivals at `rate`/s, sides 512/1024/2048/4096 (50/30/15/5 %), S = log2(side) - 5."""
     import numpy as np
     from pymoresane_amd import batch, synth
@@ -115,8 +118,8 @@ def bench_served(count=60, workers=16, rate=20.0, budget=200e9):
         inputs.append((t_arr, side, dirty))
     out = {}
     results = {}
-    for label, nw in (("plain", 1), ("served", workers)):
-        srv = batch.Server(n_workers=nw, budget_bytes=budget)
+    for label, nw in (("plain", 1), ("served", max_batch)):
+        srv = batch.Server(max_batch=nw, budget_bytes=budget)
         handles = []
         t0 = time.perf_counter()
         for t_arr, side, dirty in inputs:
@@ -131,7 +134,7 @@ def bench_served(count=60, workers=16, rate=20.0, budget=200e9):
         results[label] = [(h["model"], h["resid"]) for h in handles]
         srv.close()
         out[label] = {"images_per_s": round(len(handles) / span, 3), "p50_latency_s": round(pct(lat, 50), 4),
-                      "p99_latency_s": round(pct(lat, 99), 4), "makespan_s": round(span, 3), "workers": nw,
+                      "p99_latency_s": round(pct(lat, 99), 4), "makespan_s": round(span, 3), "max_batch": nw,
                       "all_ok": ok}
         log("c4 %s: %.2f img/s p50 %.3fs p99 %.3fs" % (label, len(handles) / span, pct(lat, 50), pct(lat, 99)))
     same = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
@@ -141,6 +144,28 @@ def bench_served(count=60, workers=16, rate=20.0, budget=200e9):
     out["workload"] = ("c4: %d Poisson arrivals at %.0f/s, sides 512/1024/2048/4096 (50/30/15/5%%), "
                        "S=log2(side)-5, HBM budget %.0f GB" % (count, rate, budget / 1e9))
     return out
+
+
+def run_precision(out_path, n=2048, stop_scale=6):
+    """2048^2 dirty / 4096^2 PSF (2000 points + 50 blobs, noise 1e-4), moresane_by_scale(1..6).
+
+    Writes model, residual and the decision trace to out_path (.npz) for comparison with the
+    float64 oracle run of the same case, and returns the timing line.
+    """
+    import numpy as np
+    from pymoresane_amd import synth
+    from pymoresane_amd.main import DataImage
+    dirty, psf = synth.config_c2(n=n)
+    img = DataImage(dirty, psf, None, 1, 1)
+    kw = dict(MORESANE_KW)
+    t0 = time.perf_counter()
+    img.moresane_by_scale(start_scale=1, stop_scale=stop_scale, **kw)
+    dt = time.perf_counter() - t0
+    np.savez(out_path, model=img.model, residual=img.residual,
+             trace=np.array(json.dumps([list(t) for t in img.trace])))
+    return {"workload": "c2 precision form: %dx%d dirty, %dx%d PSF, moresane_by_scale(1..%d)"
+                        % (n, n, 2 * n, 2 * n, stop_scale),
+            "seconds": round(dt, 3), "trace_events": len(img.trace), "output": out_path}
 
 
 def main():
@@ -153,9 +178,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched (c3) and served (c4) forms")
     ap.add_argument("--serve-requests", type=int, default=60)
-    ap.add_argument("--workers", type=int, default=16)
+    ap.add_argument("--max-batch", type=int, default=32, help="jobs per lockstep batch (batched/served forms)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB float32 copy kernel 5 times (HBM counter calibration)")
+    ap.add_argument("--precision", metavar="OUT_NPZ",
+                    help="only run the 2048^2 by-scale precision case and write model/residual/trace to OUT_NPZ")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the MORESANE hot path runs one image per device and this build "
@@ -179,6 +206,9 @@ def main():
         e1.record()
         ms = e0.elapsed_ms(e1) / 5
         print(json.dumps({"copy_bytes_each_way": 4 * count, "ms": ms, "GBps_read_plus_write": gbs(8 * count, ms)}))
+        return
+    if args.precision:
+        print(json.dumps(run_precision(args.precision)))
         return
     n, S = args.n, args.scales
     p = 2 * n
@@ -287,8 +317,8 @@ def main():
     for buf in (d_img, d_psf, det, sm, rec, out, spec, iw, fw, fw_psf):
         buf.free()
     if not args.no_batch:
-        res["batched"] = bench_batched(args.workers)
-        res["served"] = bench_served(args.serve_requests, args.workers)
+        res["batched"] = bench_batched(args.max_batch)
+        res["served"] = bench_served(args.serve_requests, args.max_batch)
     if not args.no_cpu:
         log("cpu baseline (float64 oracle, 1 core, 2048^2 sample)")
         res["cpu_baseline"] = cpu_baseline()

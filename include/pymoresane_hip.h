@@ -177,10 +177,15 @@ int pms_moresane_by_scale(const pms_moresane_params* prm, int start_scale, int s
                           pms_trace_event* trace, int trace_cap, int* trace_len, void* stream);
 
 /* ------------------------------------------------------------- batched form */
-/* Independent deconvolution jobs run concurrently: n_workers host threads,
- * each with its own stream and device workspace, take jobs in order.  Each
- * job's result is bit-identical to running it alone.  No reference
- * counterpart: the reference processes one image per call. */
+/* Independent deconvolution jobs run together.  Jobs are grouped into size
+ * classes (image side, PSF side, subregion, convolution mode); each of up to
+ * n_workers host threads takes a class and steps up to max_batch of its jobs
+ * in lockstep on its own stream, refilling slots as jobs finish: per step one
+ * launch per kernel serves every job's CG iteration, followed by one copy of
+ * the jobs' decision scalars and one synchronisation.  max_batch = 1 with one
+ * worker is the plain one-job-at-a-time path.  Each job's result is
+ * bit-identical to running it alone.  No reference counterpart: the
+ * reference processes one image per call. */
 typedef struct pms_batch_job {
   const float* dirty;
   const float* psf;
@@ -196,17 +201,21 @@ typedef struct pms_batch_job {
   int complete;     /* out */
   double seconds;   /* out: wall time of the job */
   double t_start;   /* out: start offset from the batch start, seconds */
-  double wait_seconds; /* out: host time spent waiting for the job's stream */
+  double wait_seconds; /* out: host time its runner spent waiting on the GPU while the job ran */
 } pms_batch_job;
 
-int pms_batch_run(pms_batch_job* jobs, int n_jobs, int n_workers);
+int pms_batch_run(pms_batch_job* jobs, int n_jobs, int n_workers, int max_batch);
 
 /* ------------------------------------------------------------- served form */
-/* Continuous-batching server: requests queue FIFO and are admitted, as soon
- * as a worker is free and their device footprint fits the HBM budget, onto
- * one of n_workers streams, overlapping with requests already running.  Host
- * buffers must stay valid until the request is done.  Times are seconds since
- * the server was created.  No reference counterpart: the reference has no
+/* Continuous-batching server: requests join the FIFO queue of their size
+ * class; each class has a host thread and stream stepping up to max_batch
+ * admitted requests in lockstep (as the batched form does).  A queued request
+ * is admitted between steps as soon as its class has a free slot and its
+ * device footprint fits the HBM budget shared by all classes, so requests that
+ * arrive while others run join the running batch.  Host buffers must stay
+ * valid until the request is done.  max_batch = 1 is the plain path: one
+ * lane for all classes, one request at a time.  Times are seconds since the
+ * server was created.  No reference counterpart: the reference has no
  * serving path. */
 typedef struct pms_serve_request {
   const float* dirty_host;  /* n x n */
@@ -223,7 +232,7 @@ typedef struct pms_serve_request {
   double t_submit, t_start, t_done;
 } pms_serve_request;
 
-int pms_server_create(int n_workers, double budget_bytes, void** server);
+int pms_server_create(int max_batch, double budget_bytes, void** server);
 int pms_server_submit(void* server, pms_serve_request* req);
 int pms_server_wait(void* server, pms_serve_request* req, double timeout_s, int* done);
 int pms_server_drain(void* server);

@@ -97,7 +97,7 @@ SIGNATURES = {
                              c_int_p, ctypes.POINTER(TraceEvent), c_int, c_int_p, c_void_p]),
     "pms_moresane_by_scale": (c_int, [ctypes.POINTER(MoresaneParams), c_int, c_int, c_void_p, c_void_p, c_void_p,
                                       c_void_p, c_void_p, ctypes.POINTER(TraceEvent), c_int, c_int_p, c_void_p]),
-    "pms_batch_run": (c_int, [ctypes.POINTER(BatchJob), c_int, c_int]),
+    "pms_batch_run": (c_int, [ctypes.POINTER(BatchJob), c_int, c_int, c_int]),
     "pms_server_create": (c_int, [c_int, c_double, ctypes.POINTER(c_void_p)]),
     "pms_server_submit": (c_int, [c_void_p, ctypes.POINTER(ServeRequest)]),
     "pms_server_wait": (c_int, [c_void_p, ctypes.POINTER(ServeRequest), c_double, c_int_p]),

@@ -1,11 +1,11 @@
 """Batched and served MORESANE on one MI355X.
 
-* ``run_batch`` -- many independent dirty/PSF pairs (ragged sizes) run
-  concurrently by the library's worker pool, one HIP stream and one
-  grow-only device workspace per worker, inputs resident in HBM.
-* ``Server``    -- continuous batching: requests are queued and admitted by an
-  HBM-footprint budget onto worker streams as workers and memory free up,
-  overlapping with requests already on the GPU.
+* ``run_batch`` -- many independent dirty/PSF pairs (ragged sizes), inputs
+  resident in HBM: jobs of one size class are stepped in lockstep, so every
+  kernel of a CG iteration is one launch for the whole batch; classes run
+  concurrently on their own streams.
+* ``Server``    -- continuous batching: requests queue per size class and join
+  the class's running batch between steps as slots and HBM budget free up.
 Both produce, per job, exactly the result of running that job alone.
 """
 
@@ -57,8 +57,13 @@ class DeviceJobs(object):
         return [(it["model"].get(), it["resid"].get()) for it in self.items]
 
 
-def run_batch(jobs, n_workers=8, order=None, by_scale=False, start_scale=1, stop_scale=20):
-    """Run every job of a DeviceJobs set; returns per-job (status, major_iters, seconds, t_start)."""
+def run_batch(jobs, n_workers=4, max_batch=32, order=None, by_scale=False, start_scale=1, stop_scale=20):
+    """Run every job of a DeviceJobs set; returns per-job (status, major_iters, seconds, t_start, wait_seconds).
+
+    Jobs are grouped by size class; up to ``n_workers`` classes run at once, each stepping up to
+    ``max_batch`` jobs in lockstep with batched launches.  ``n_workers=1, max_batch=1`` is the
+    plain one-job-at-a-time path.
+    """
     lib = _lib.lib()
     idx = list(range(len(jobs.items))) if order is None else list(order)
     arr = (BatchJob * len(idx))()
@@ -69,7 +74,7 @@ def run_batch(jobs, n_workers=8, order=None, by_scale=False, start_scale=1, stop
         b.model, b.residual = it["model"].ptr, it["resid"].ptr
         b.prm = it["prm"]
         b.by_scale, b.start_scale, b.stop_scale = int(by_scale), int(start_scale), int(stop_scale)
-    check(lib.pms_batch_run(arr, len(idx), int(n_workers)))
+    check(lib.pms_batch_run(arr, len(idx), int(n_workers), int(max_batch)))
     out = [None] * len(jobs.items)
     for slot, i in enumerate(idx):
         b = arr[slot]
@@ -80,12 +85,13 @@ def run_batch(jobs, n_workers=8, order=None, by_scale=False, start_scale=1, stop
 
 
 class Server(object):
-    """Continuous-batching MORESANE server over the library's worker pool."""
+    """Continuous-batching MORESANE server: one lockstep batch of up to ``max_batch`` requests per
+    size class, requests admitted between steps under a shared HBM budget."""
 
-    def __init__(self, n_workers=8, budget_bytes=200e9):
+    def __init__(self, max_batch=32, budget_bytes=200e9):
         self._lib = _lib.lib()
         self._h = ctypes.c_void_p(0)
-        check(self._lib.pms_server_create(int(n_workers), float(budget_bytes), ctypes.byref(self._h)))
+        check(self._lib.pms_server_create(int(max_batch), float(budget_bytes), ctypes.byref(self._h)))
         self._live = []
 
     def now(self):

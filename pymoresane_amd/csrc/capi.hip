@@ -1,10 +1,12 @@
 // extern "C" entry points of libpmship.so (declared in include/pymoresane_hip.h).
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <thread>
+#include <tuple>
 
 #include "engine.h"
 
@@ -50,18 +52,41 @@ __global__ __launch_bounds__(256) void k_diffsq(const float* __restrict__ a, con
 }
 }  // namespace
 
-// one cached Job per stream (device buffers reused across calls of the same shape)
-static std::mutex g_jobs_mu;
-static std::map<std::pair<int, hipStream_t>, std::unique_ptr<Job>> g_jobs;
+// one cached single-slot runner per (device, stream): device buffers reused across calls of the same
+// shape.  A NULL stream gets a runner-owned stream (graph capture needs a stream of its own).
+static std::mutex g_runners_mu;
+static std::map<std::pair<int, hipStream_t>, std::unique_ptr<Runner>> g_runners;
 
-static Job& job_for(hipStream_t st) {
+static Runner& runner_for(hipStream_t st) {
   int dev = 0;
   PMS_HIP(hipGetDevice(&dev));
-  std::lock_guard<std::mutex> lk(g_jobs_mu);
+  std::lock_guard<std::mutex> lk(g_runners_mu);
   auto key = std::make_pair(dev, st);
-  auto it = g_jobs.find(key);
-  if (it == g_jobs.end()) it = g_jobs.emplace(key, std::make_unique<Job>(st)).first;
+  auto it = g_runners.find(key);
+  if (it == g_runners.end()) it = g_runners.emplace(key, std::make_unique<Runner>(1, st)).first;
   return *it->second;
+}
+
+// Run one task to completion on a single-slot runner; the task's error (if any) is rethrown after
+// its trace has been copied out.
+static void run_single(Runner& rn, const TaskSpec& spec, Trace& tr, int* complete, int* major) {
+  const int slot = rn.add(spec);
+  PMS_REQUIRE(slot >= 0, "runner busy");
+  try {
+    rn.run();
+  } catch (...) {
+    tr = rn.task(slot)->trace;
+    rn.release(slot);
+    throw;
+  }
+  Task* t = rn.task(slot);
+  tr = t->trace;
+  const int status = t->status;
+  const std::string err = t->error;
+  if (complete) *complete = t->complete;
+  if (major) *major = t->major_iters;
+  rn.release(slot);
+  if (status != 0) fail(status, err);
 }
 
 static void copy_trace(const Trace& tr, pms_trace_event* out, int cap, int* len) {
@@ -168,10 +193,10 @@ int pms_stream_create(void** stream) {
 int pms_stream_destroy(void* stream) {
   return guarded([&] {
     {
-      std::lock_guard<std::mutex> lk(g_jobs_mu);
-      for (auto it = g_jobs.begin(); it != g_jobs.end();) {
+      std::lock_guard<std::mutex> lk(g_runners_mu);
+      for (auto it = g_runners.begin(); it != g_runners.end();) {
         if (it->first.second == S(stream))
-          it = g_jobs.erase(it);
+          it = g_runners.erase(it);
         else
           ++it;
       }
@@ -331,9 +356,9 @@ int pms_moresane(const pms_moresane_params* prm, const float* dirty, const float
     PMS_REQUIRE(dirty && psf && model_inout && residual_inout, "NULL buffer");
     Trace tr;
     int complete = 0;
-    Job& job = job_for(S(stream));
+    const TaskSpec spec{*prm, dirty, psf, mask, model_inout, residual_inout, 0, 0, 0};
     try {
-      job.moresane(*prm, prm->scale_count, dirty, psf, mask, model_inout, residual_inout, &complete, tr);
+      run_single(runner_for(S(stream)), spec, tr, &complete, nullptr);
     } catch (...) {
       copy_trace(tr, trace, trace_cap, trace_len);
       throw;
@@ -350,9 +375,9 @@ int pms_moresane_by_scale(const pms_moresane_params* prm, int start_scale, int s
     check_params(prm);
     PMS_REQUIRE(dirty && psf && model_inout && residual_inout, "NULL buffer");
     Trace tr;
-    Job& job = job_for(S(stream));
+    const TaskSpec spec{*prm, dirty, psf, mask, model_inout, residual_inout, 1, start_scale, stop_scale};
     try {
-      job.moresane_by_scale(*prm, start_scale, stop_scale, dirty, psf, mask, model_inout, residual_inout, tr);
+      run_single(runner_for(S(stream)), spec, tr, nullptr, nullptr);
     } catch (...) {
       copy_trace(tr, trace, trace_cap, trace_len);
       throw;
@@ -362,53 +387,89 @@ int pms_moresane_by_scale(const pms_moresane_params* prm, int start_scale, int s
 }
 
 // ---------------------------------------------------------------------- batch
-int pms_batch_run(pms_batch_job* jobs, int n_jobs, int n_workers) {
+int pms_batch_run(pms_batch_job* jobs, int n_jobs, int n_workers, int max_batch) {
   return guarded([&] {
     PMS_REQUIRE(n_jobs >= 0 && (n_jobs == 0 || jobs), "bad job list");
+    PMS_REQUIRE(max_batch >= 1 && max_batch <= 1024, "max_batch must be in 1..1024");
     if (n_jobs == 0) return;
     if (n_workers < 1) n_workers = 1;
-    if (n_workers > n_jobs) n_workers = n_jobs;
+    // size classes, largest total work first; jobs keep their order inside a class
+    std::map<std::tuple<int, int, int, int>, std::vector<int>> by_class;
+    for (int i = 0; i < n_jobs; ++i) {
+      const pms_moresane_params& p = jobs[i].prm;
+      const int sub = (p.subregion <= 0 || p.subregion > p.n) ? p.n : p.subregion;
+      by_class[std::make_tuple(p.n, p.psf_n, sub, p.conv_mode)].push_back(i);
+    }
+    std::vector<std::vector<int>> classes;
+    for (auto& kv : by_class) classes.push_back(kv.second);
+    std::stable_sort(classes.begin(), classes.end(), [&](const std::vector<int>& a, const std::vector<int>& b) {
+      const double wa = (double)jobs[a[0]].prm.n * jobs[a[0]].prm.n * a.size();
+      const double wb = (double)jobs[b[0]].prm.n * jobs[b[0]].prm.n * b.size();
+      return wa > wb;
+    });
+    if (n_workers > (int)classes.size()) n_workers = (int)classes.size();
     int dev = 0;
     PMS_HIP(hipGetDevice(&dev));
-    std::atomic<int> next{0};
+    std::atomic<int> next_class{0};
     const auto t0 = std::chrono::steady_clock::now();
+    auto since = [&](std::chrono::steady_clock::time_point t) {
+      return std::chrono::duration<double>(t - t0).count();
+    };
     std::vector<std::thread> pool;
     std::vector<std::string> errs(n_workers);
     for (int wk = 0; wk < n_workers; ++wk) {
       pool.emplace_back([&, wk] {
-        hipStream_t st = nullptr;
-        if (hipSetDevice(dev) != hipSuccess || hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
-          errs[wk] = "worker could not create a stream";
-          return;
-        }
-        {
-          Job job(st);
+        try {
+          PMS_HIP(hipSetDevice(dev));
+          Runner rn(max_batch);
+          std::vector<int> slot_job(max_batch, -1);
+          std::vector<double> slot_wait0(max_batch, 0.0);
           for (;;) {
-            const int i = next.fetch_add(1);
-            if (i >= n_jobs) break;
-            pms_batch_job& J = jobs[i];
-            const auto a = std::chrono::steady_clock::now();
-            J.t_start = std::chrono::duration<double>(a - t0).count();
-            Trace tr;
-            job.reset_wait();
-            J.status = guarded([&] {
-              check_params(&J.prm);
-              if (J.by_scale) {
-                job.moresane_by_scale(J.prm, J.start_scale, J.stop_scale, J.dirty, J.psf, J.mask, J.model, J.residual,
-                                      tr);
-                J.complete = 0;
-              } else {
-                int complete = 0;
-                J.major_iters = job.moresane(J.prm, J.prm.scale_count, J.dirty, J.psf, J.mask, J.model, J.residual,
-                                             &complete, tr);
-                J.complete = complete;
+            const int c = next_class.fetch_add(1);
+            if (c >= (int)classes.size()) break;
+            const std::vector<int>& q = classes[c];
+            size_t qi = 0;
+            for (;;) {
+              // refill free slots from this class, then step
+              for (int k = rn.first_free(); k >= 0 && qi < q.size(); k = rn.first_free()) {
+                pms_batch_job& J = jobs[q[qi++]];
+                J.t_start = since(std::chrono::steady_clock::now());
+                J.status = guarded([&] {
+                  check_params(&J.prm);
+                  const TaskSpec spec{J.prm,   J.dirty, J.psf, J.mask, J.model, J.residual, J.by_scale,
+                                      J.start_scale, J.stop_scale};
+                  rn.add(spec, k);
+                });
+                if (J.status != 0) {
+                  J.seconds = 0.0;
+                  continue;
+                }
+                slot_job[k] = (int)(&J - jobs);
+                slot_wait0[k] = rn.wait_seconds();
               }
-            });
-            J.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
-            J.wait_seconds = job.wait_seconds();
+              if (rn.idle()) break;
+              rn.step();
+              const double now = since(std::chrono::steady_clock::now());
+              for (int k = 0; k < max_batch; ++k) {
+                Task* t = rn.task(k);
+                if (!t || !t->done()) continue;
+                pms_batch_job& J = jobs[slot_job[k]];
+                J.status = t->status;
+                if (t->status != 0) set_error(t->error);
+                J.major_iters = t->major_iters;
+                J.complete = J.by_scale ? 0 : t->complete;
+                J.seconds = now - J.t_start;
+                J.wait_seconds = rn.wait_seconds() - slot_wait0[k];
+                rn.release(k);
+                slot_job[k] = -1;
+              }
+            }
           }
+        } catch (const Error& e) {
+          errs[wk] = e.msg;
+        } catch (const std::exception& e) {
+          errs[wk] = e.what();
         }
-        hipStreamDestroy(st);
       });
     }
     for (auto& t : pool) t.join();

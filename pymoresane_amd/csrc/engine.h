@@ -1,7 +1,10 @@
-// Device-resident MORESANE driver: per-job state and the major/minor loops.
+// Device-resident MORESANE driver: per-job workspace, the major/minor loops as a
+// resumable state machine, and a runner that steps many of them in lockstep.
 #pragma once
 
 #include <map>
+#include <memory>
+#include <string>
 #include <tuple>
 #include <vector>
 
@@ -55,48 +58,27 @@ struct JobSizes {
 // device bytes one job of this geometry needs (its workspace plus nothing else)
 size_t job_footprint_bytes(const Geometry& g);
 
-// Buffers and scales of one CG iteration: the key of a captured iteration graph.
-struct IterArgs {
-  float *x, *xn, *r, *rn, *p, *ap;
-  double *rr, *rnrn;
-  int max_scale, adj, enforce, sub, p_sub;
-  bool operator<(const IterArgs& o) const {
-    return std::tie(x, xn, r, rn, p, ap, rr, rnrn, max_scale, adj, enforce, sub, p_sub) <
-           std::tie(o.x, o.xn, o.r, o.rn, o.p, o.ap, o.rr, o.rnrn, o.max_scale, o.adj, o.enforce, o.sub, o.p_sub);
-  }
-};
+// Per-job scalar slot (doubles) in the runner's scalar block: the only values read back.
+constexpr int kSlotDoubles = 64;
+constexpr int kSlotInts = 64;
 
-// Device buffers needed by one moresane() call, kept across calls of the same shape.
+// Device buffers of one deconvolution, kept across calls of the same or smaller shape.
 class Job {
  public:
-  explicit Job(hipStream_t st);
+  Job() = default;
   ~Job();
-  int moresane(const MoresaneParams& prm, int scale_count, const float* dirty_dev, const float* psf_dev,
-               const float* mask_dev, float* model_dev, float* residual_dev, int* complete, Trace& trace);
-  void moresane_by_scale(const MoresaneParams& prm, int start_scale, int stop_scale, const float* dirty_dev,
-                         const float* psf_dev, const float* mask_dev, float* model_dev, float* residual_dev,
-                         Trace& trace);
-  size_t device_bytes() const { return arena_.bytes(); }
-  double wait_seconds() const { return wait_seconds_; }
-  void reset_wait() { wait_seconds_ = 0.0; }
-  hipStream_t stream() const { return st_; }
+  Job(const Job&) = delete;
+  Job& operator=(const Job&) = delete;
+  // grow-only allocation for geometry g; returns true when buffers moved
+  bool ensure(const Geometry& g, hipStream_t st);
+  float* by_scale_dirty(size_t n2);  // private copy of the dirty image for moresane_by_scale
+  size_t device_bytes() const { return arena_.bytes() + by_scale_arena_.bytes(); }
+  const Geometry& geometry() const { return geo_; }
 
  private:
-  void ensure(const Geometry& g);
-  void setup_psf(const float* psf_dev, std::vector<double>& energies);
-  void apply_op_masked(const float* v, float* out, int max_scale, int adj, const int* pred = nullptr);
-  double residual_and_std(const float* dirty_dev, const float* model_dev);
-  void enqueue_iteration(const IterArgs& a);
-  void run_iteration(const IterArgs& a, bool use_graph);
-  void drop_graphs();
-  void sync();
-  double wait_seconds_ = 0.0;
-  std::map<IterArgs, hipGraphExec_t> graphs_;
-
-  hipStream_t st_;
-  bool owns_stream_ = false;
   DeviceArena arena_;
   DeviceArena by_scale_arena_;
+  size_t by_scale_n2_ = 0;
   Geometry geo_{};
   JobSizes cap_{};
   bool have_ = false;
@@ -113,9 +95,145 @@ class Job {
   void *mad_work = nullptr, *ext_work = nullptr;
   double* partial = nullptr;
   float* pmax = nullptr;
-  double* scal = nullptr;
-  int* iflag = nullptr;
-  int* border_dev = nullptr;
+  float* cur_dirty = nullptr;
+};
+
+// One request: moresane (main.py:53-521) or moresane_by_scale (main.py:523-599) of one image.
+struct TaskSpec {
+  MoresaneParams prm;
+  const float* dirty;
+  const float* psf;
+  const float* mask;  // may be null
+  float* model;       // accumulated into
+  float* residual;    // overwritten when a major iteration ran
+  int by_scale, start_scale, stop_scale;
+};
+
+class Runner;
+
+// Buffers and scales of one CG iteration of one job (also the key of a captured iteration graph).
+struct IterArgs {
+  float *x, *xn, *r, *rn, *p, *ap;
+  double *rr, *rnrn;
+  int max_scale, adj, enforce, sub, p_sub;
+  bool operator<(const IterArgs& o) const {
+    return std::tie(x, xn, r, rn, p, ap, rr, rnrn, max_scale, adj, enforce, sub, p_sub) <
+           std::tie(o.x, o.xn, o.r, o.rn, o.p, o.ap, o.rr, o.rnrn, o.max_scale, o.adj, o.enforce, o.sub, o.p_sub);
+  }
+};
+
+// The MORESANE control flow of main.py:53-599 as a state machine.  Each step the task enqueues the
+// GPU work of its phase on the runner's stream; after the runner's one synchronisation the task
+// reads its scalar slot and runs the host decisions up to its next GPU phase.
+class Task {
+ public:
+  enum Phase { SETUP, DECOMP, MINOR, UPDATE, REVERT, FINISH, DONE };
+  Task(const TaskSpec& spec, Job* job, double* scal, int* iflag, double* upload_host);
+
+  Phase phase() const { return phase_; }
+  bool done() const { return phase_ == DONE; }
+  void enqueue(hipStream_t st);                     // GPU work of the current phase (MINOR: start only)
+  bool wants_iteration() const { return phase_ == MINOR && iterate_; }
+  IterArgs iter_args() const;
+  void consume(const double* slot_host);            // host decisions after the step's sync
+  void fail(int code, const std::string& msg);
+
+  Job* job() const { return job_; }
+  double* scal() const { return scal_; }
+  int* iflag() const { return iflag_; }
+  bool geometry_moved() const { return moved_; }
+
+  TaskSpec spec;
+  Trace trace;
+  int status = 0;
+  std::string error;
+  int major_iters = 0;  // major iterations of the last moresane call
+  int complete = 0;
+  int steps = 0;
+
+ private:
+  void start_call(int scale_count);
+  void major_check();
+  void inner_check();
+  void choose_scales();
+  void minor_exit();
+  void after_inner();
+  void enqueue_setup(hipStream_t st);
+  void enqueue_decomp(hipStream_t st);
+  void enqueue_minor_start(hipStream_t st);
+  void enqueue_update(hipStream_t st, double gain);
+  void enqueue_finish(hipStream_t st);
+  const float* dirty_dev() const { return spec.by_scale ? job_->cur_dirty : spec.dirty; }
+
+  Job* job_;
+  double* scal_;          // device slot (kSlotDoubles doubles)
+  int* iflag_;            // device slot (kSlotInts ints, zero between kernels)
+  double* upload_;        // pinned host staging for this slot's small uploads
+  Phase phase_ = SETUP;
+  bool moved_ = false;
+  bool by_scale_copied_ = false;
+
+  // per moresane call
+  Geometry g_{};
+  int scale_count_ = 0;
+  NoiseMask nm_{};
+  std::vector<int> borders_;
+  int upload_energies_ = 0;  // energies waiting in upload_ for the next enqueue
+  int major_ = 0, min_scale_ = 0, max_scale_ = 0, scale_adjust_ = 0;
+  double max_coeff_ = 1.0, std_current_ = 1000.0, std_ratio_ = 1.0;
+  float dmax_ = 0.f;
+  std::vector<double> nsm_;
+  // minor loop
+  bool minor_first_ = false, iterate_ = false;
+  int niter_ = 0, exit_code_ = EXIT_CAP;
+  double e2_ = 0.0, snr_last_ = 0.0, snr_current_ = 0.0;
+  float *x_ = nullptr, *xn_ = nullptr, *r_ = nullptr, *rn_ = nullptr;
+  double *rr_ = nullptr, *rnrn_ = nullptr;
+};
+
+// Steps a set of tasks in lockstep on one stream: per step every task enqueues its phase's work,
+// the CG iterations of all tasks that share a plane shape run as batched launches (one launch per
+// kernel for up to kMaxBatch jobs), then one copy of the scalar block and one synchronisation.
+// Each task's arithmetic is independent of the others, so a task gives the same result alone and
+// in a batch.
+class Runner {
+ public:
+  explicit Runner(int slots, hipStream_t st = nullptr);
+  ~Runner();
+  Runner(const Runner&) = delete;
+  Runner& operator=(const Runner&) = delete;
+
+  int slots() const { return (int)jobs_.size(); }
+  int free_slots() const;
+  // start a task in a free slot (the given one, or the first free); returns the slot or -1
+  int add(const TaskSpec& spec, int slot = -1);
+  int first_free() const;
+  Task* task(int slot) { return tasks_[slot].get(); }
+  void release(int slot);  // drop a finished task (its workspace is kept)
+  bool idle() const;
+  void step();                 // one lockstep step (no-op when idle)
+  void run();                  // step until every task is done
+  hipStream_t stream() const { return st_; }
+  double wait_seconds() const { return wait_seconds_; }
+  size_t device_bytes() const;
+
+ private:
+  void run_iterations(std::vector<Task*>& ts);
+  void enqueue_iterations(const std::vector<IterArgs>& its, const std::vector<Task*>& ts);
+  void apply_op(const std::vector<Task*>& ts, const std::vector<IterArgs>& its, bool clip_pass);
+  void drop_graphs();
+
+  hipStream_t st_;
+  bool owns_stream_ = false;
+  std::vector<std::unique_ptr<Job>> jobs_;
+  std::vector<std::unique_ptr<Task>> tasks_;
+  double* scal_dev_ = nullptr;
+  int* iflag_dev_ = nullptr;
+  double* scal_host_ = nullptr;    // pinned readback mirror
+  double* upload_host_ = nullptr;  // pinned upload staging, kSlotDoubles per slot
+  double wait_seconds_ = 0.0;
+  bool use_graph_ = false;
+  std::map<std::vector<IterArgs>, hipGraphExec_t> graphs_;
 };
 
 }  // namespace pms

@@ -1,5 +1,6 @@
 // Device-resident MORESANE driver (major loop, conjugate-gradient minor loop,
-// by-scale outer loop) for one job on one HIP stream.
+// by-scale outer loop), written as a per-job state machine that a Runner
+// steps for one or many jobs at once.
 //
 // Control flow follows pymoresane/main.py:53-599 step for step; every array
 // stays in HBM and only the per-iteration decision scalars (normalised scale
@@ -12,13 +13,16 @@
 //   * CG minor loop (main.py:365-457): vectors in float32, every inner
 //     product, step size and norm in float64; the positivity clip branch is
 //     predicated on a device flag so no extra synchronisation happens;
-//     one host sync per iteration reads the SNR sums for the exit rules;
+//     the host reads the SNR sums once per iteration for the exit rules;
 //   * model update and residual (main.py:479-506), with revert.
-// Kernels run with a null predicate here; the predicate mechanism exists so a
-// caller can enqueue work whose execution depends on device state.
+// The Runner gives each job a 64-double scalar slot in one block, so a step
+// of any number of jobs ends with one copy of that block and one
+// synchronisation, and the CG iterations of jobs that share a plane shape
+// run as batched launches (one launch per kernel for up to kMaxBatch jobs).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "engine.h"
@@ -28,16 +32,106 @@ namespace {
 
 constexpr int kT = 256;
 
+// Scalar slot layout (doubles): [0,16) thresholds, [16,32) normalised maxima, [32,48) PSF scale
+// energies, 48/51 rr and rnrn (swapped every iteration), 49 p.Ap, 50 alpha, 52 beta, 53 SNR
+// residual sum, 54 SNR energy sum, 56/57 mean sum and squared deviations, float at 60: dmax.
+// Int slot: [0] clip flag (written only for jobs that enforce positivity), [16,32) suppression borders.
+constexpr int kThr = 0, kNsm = 16, kEnergy = 32, kRR = 48, kPAP = 49, kAlpha = 50, kRNRN = 51, kBeta = 52,
+              kD2 = 53, kE2 = 54, kMeanSum = 56, kDev2 = 57, kDmax = 60;
+constexpr int kClip = 0, kBorders = 16;
 
+// Per-job pointers of one batched CG launch (grid y selects the job).
+struct CgJob {
+  float *x, *xn, *r, *rn, *p, *ap;
+  double *partial, *scal, *rr, *rnrn;
+  int* iflag;
+  int enforce;
+};
+struct CgBatch {
+  CgJob j[kMaxBatch];
+  size_t n;
+};
 
+// ---- CG vector kernels.  Planes hold a multiple of 4 floats (the side is even), so each thread
+// walks float4 chunks.  Reductions write one partial per block; a one-block-per-job finishing
+// kernel adds the partials in block order (kernel boundaries order the two, no fences needed).
+__device__ __forceinline__ double block_total(const double* __restrict__ partial, int np) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += kT) acc += partial[i];
+  return block_sum_f64(acc);
+}
+
+// per-block partials of p . Ap
+__global__ __launch_bounds__(kT) void k_cg_pap(CgBatch b) {
+  const CgJob& J = b.j[blockIdx.y];
+  const float4* __restrict__ p = reinterpret_cast<const float4*>(J.p);
+  const float4* __restrict__ ap = reinterpret_cast<const float4*>(J.ap);
+  const size_t n4 = b.n >> 2;
+  double acc = 0.0;
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n4; i += (size_t)gridDim.x * kT) {
+    const float4 u = p[i], v = ap[i];
+    acc = fma((double)u.x, (double)v.x, acc);
+    acc = fma((double)u.y, (double)v.y, acc);
+    acc = fma((double)u.z, (double)v.z, acc);
+    acc = fma((double)u.w, (double)v.w, acc);
+  }
+  acc = block_sum_f64(acc);
+  if (threadIdx.x == 0) J.partial[blockIdx.x] = acc;
+}
+
+// alpha = rr / (p . Ap)
+__global__ __launch_bounds__(kT) void k_cg_alpha_fin(CgBatch b, int np) {
+  const CgJob& J = b.j[blockIdx.y];
+  const double tot = block_total(J.partial, np);
+  if (threadIdx.x == 0) {
+    J.scal[kPAP] = tot;
+    J.scal[kAlpha] = *J.rr / tot;
+  }
+}
+
+__device__ __forceinline__ float xn_of(double a, float p, float x) { return (float)fma(a, (double)p, (double)x); }
+
+// xn = x + alpha p ; per-block minima of xn
+__global__ __launch_bounds__(kT) void k_cg_xn(CgBatch b) {
+  const CgJob& J = b.j[blockIdx.y];
+  const double a = J.scal[kAlpha];
+  const float4* __restrict__ x = reinterpret_cast<const float4*>(J.x);
+  const float4* __restrict__ p = reinterpret_cast<const float4*>(J.p);
+  float4* __restrict__ xn = reinterpret_cast<float4*>(J.xn);
+  const size_t n4 = b.n >> 2;
+  float m = INFINITY;
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n4; i += (size_t)gridDim.x * kT) {
+    const float4 u = p[i], v = x[i];
+    float4 o;
+    o.x = xn_of(a, u.x, v.x);
+    o.y = xn_of(a, u.y, v.y);
+    o.z = xn_of(a, u.z, v.z);
+    o.w = xn_of(a, u.w, v.w);
+    xn[i] = o;
+    m = fminf(m, fminf(fminf(o.x, o.y), fminf(o.z, o.w)));
+  }
+  m = block_min_f32(m);
+  if (threadIdx.x == 0) J.partial[blockIdx.x] = (double)m;
+}
+
+// clip flag = enforce && min(xn) < 0
+__global__ __launch_bounds__(kT) void k_cg_flag_fin(CgBatch b, int np) {
+  const CgJob& J = b.j[blockIdx.y];
+  float m = INFINITY;
+  for (int i = threadIdx.x; i < np; i += kT) m = fminf(m, (float)J.partial[i]);
+  m = block_min_f32(m);
+  if (threadIdx.x == 0) J.iflag[kClip] = (J.enforce && m < 0.f) ? 1 : 0;
+}
 
 // positivity clip: xn[xn<0] = 0 ; p = (xn - x) / alpha
-__global__ __launch_bounds__(kT) void k_cg_clip(float* __restrict__ xn, const float* __restrict__ x,
-                                                float* __restrict__ p, const double* __restrict__ alpha, size_t n,
-                                                const int* flag) {
-  if (*flag == 0) return;
-  const double a = *alpha;
-  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) {
+__global__ __launch_bounds__(kT) void k_cg_clip(CgBatch b) {
+  const CgJob& J = b.j[blockIdx.y];
+  if (J.iflag[kClip] == 0) return;
+  const double a = J.scal[kAlpha];
+  float* __restrict__ xn = J.xn;
+  const float* __restrict__ x = J.x;
+  float* __restrict__ p = J.p;
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < b.n; i += (size_t)gridDim.x * kT) {
     float v = xn[i];
     if (v < 0.f) v = 0.f;
     xn[i] = v;
@@ -45,13 +139,60 @@ __global__ __launch_bounds__(kT) void k_cg_clip(float* __restrict__ xn, const fl
   }
 }
 
+__device__ __forceinline__ float rn_of(double a, float ap, float r) { return (float)fma(-a, (double)ap, (double)r); }
+
+// rn = r - alpha Ap ; per-block partials of rn . rn
+__global__ __launch_bounds__(kT) void k_cg_rn(CgBatch b) {
+  const CgJob& J = b.j[blockIdx.y];
+  const double a = J.scal[kAlpha];
+  const float4* __restrict__ r = reinterpret_cast<const float4*>(J.r);
+  const float4* __restrict__ ap = reinterpret_cast<const float4*>(J.ap);
+  float4* __restrict__ rn = reinterpret_cast<float4*>(J.rn);
+  const size_t n4 = b.n >> 2;
+  double acc = 0.0;
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n4; i += (size_t)gridDim.x * kT) {
+    const float4 u = ap[i], v = r[i];
+    float4 o;
+    o.x = rn_of(a, u.x, v.x);
+    o.y = rn_of(a, u.y, v.y);
+    o.z = rn_of(a, u.z, v.z);
+    o.w = rn_of(a, u.w, v.w);
+    rn[i] = o;
+    acc = fma((double)o.x, (double)o.x, acc);
+    acc = fma((double)o.y, (double)o.y, acc);
+    acc = fma((double)o.z, (double)o.z, acc);
+    acc = fma((double)o.w, (double)o.w, acc);
+  }
+  acc = block_sum_f64(acc);
+  if (threadIdx.x == 0) J.partial[blockIdx.x] = acc;
+}
+
+// rnrn = rn . rn ; beta = rnrn / rr
+__global__ __launch_bounds__(kT) void k_cg_beta_fin(CgBatch b, int np) {
+  const CgJob& J = b.j[blockIdx.y];
+  const double tot = block_total(J.partial, np);
+  if (threadIdx.x == 0) {
+    *J.rnrn = tot;
+    J.scal[kBeta] = tot / *J.rr;
+  }
+}
 
 // p = rn + beta p
-__global__ __launch_bounds__(kT) void k_cg_p(const float* __restrict__ rn, float* __restrict__ p,
-                                             const double* __restrict__ beta, size_t n) {
-  const double b = *beta;
-  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT)
-    p[i] = (float)fma(b, (double)p[i], (double)rn[i]);
+__global__ __launch_bounds__(kT) void k_cg_p(CgBatch b) {
+  const CgJob& J = b.j[blockIdx.y];
+  const double be = J.scal[kBeta];
+  const float4* __restrict__ rn = reinterpret_cast<const float4*>(J.rn);
+  float4* __restrict__ p = reinterpret_cast<float4*>(J.p);
+  const size_t n4 = b.n >> 2;
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n4; i += (size_t)gridDim.x * kT) {
+    const float4 u = rn[i];
+    float4 v = p[i];
+    v.x = (float)fma(be, (double)v.x, (double)u.x);
+    v.y = (float)fma(be, (double)v.y, (double)u.y);
+    v.z = (float)fma(be, (double)v.z, (double)u.z);
+    v.w = (float)fma(be, (double)v.w, (double)u.w);
+    p[i] = v;
+  }
 }
 
 // model[r0+r, c0+c] += gain * x[r, c]
@@ -76,91 +217,6 @@ __global__ __launch_bounds__(kT) void k_add_inplace(float* __restrict__ a, const
   for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) a[i] += b[i];
 }
 
-// Deterministic grid-wide finish: every block publishes its partial, the last block to arrive sums
-// the partials in block order (fixed tree) and runs `fin` on the total; the counter is left at 0.
-template <class Fin>
-__device__ __forceinline__ void finish_sum(double part, double* partial, unsigned* counter, Fin fin) {
-  __shared__ int is_last;
-  if (threadIdx.x == 0) {
-    partial[blockIdx.x] = part;
-    __threadfence();
-    is_last = (atomicAdd(counter, 1u) == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  double acc = 0.0;
-  for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x) acc += ((volatile const double*)partial)[i];
-  acc = block_sum_f64(acc);
-  if (threadIdx.x == 0) {
-    fin(acc);
-    *counter = 0u;
-  }
-}
-
-// alpha = rr / (p . Ap)
-__global__ __launch_bounds__(kT) void k_cg_alpha(const float* __restrict__ p, const float* __restrict__ ap, size_t n,
-                                                 double* partial, unsigned* counter, const double* rr, double* pap,
-                                                 double* alpha) {
-  double acc = 0.0;
-  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT)
-    acc = fma((double)p[i], (double)ap[i], acc);
-  acc = block_sum_f64(acc);
-  finish_sum(acc, partial, counter, [&](double tot) {
-    *pap = tot;
-    *alpha = *rr / tot;
-  });
-}
-
-// xn = x + alpha p ; clip flag = enforce && min(xn) < 0
-__global__ __launch_bounds__(kT) void k_cg_xn_flag(const float* __restrict__ x, const float* __restrict__ p,
-                                                   const double* __restrict__ alpha, float* __restrict__ xn, size_t n,
-                                                   double* partial, unsigned* counter, int enforce, int* flag) {
-  const double a = *alpha;
-  float m = INFINITY;
-  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) {
-    const float v = (float)fma(a, (double)p[i], (double)x[i]);
-    xn[i] = v;
-    m = fminf(m, v);
-  }
-  m = block_min_f32(m);
-  // min via the same last-block pattern: publish min as a double, reduce with min in block order
-  __shared__ int is_last;
-  if (threadIdx.x == 0) {
-    partial[blockIdx.x] = (double)m;
-    __threadfence();
-    is_last = (atomicAdd(counter, 1u) == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  if (threadIdx.x == 0) {
-    double mn = INFINITY;
-    for (unsigned i = 0; i < gridDim.x; ++i) mn = fmin(mn, ((volatile const double*)partial)[i]);
-    *flag = (enforce && mn < 0.0) ? 1 : 0;
-    *counter = 0u;
-  }
-}
-
-// rn = r - alpha Ap ; rnrn = rn . rn ; beta = rnrn / rr
-__global__ __launch_bounds__(kT) void k_cg_rn_beta(const float* __restrict__ r, const float* __restrict__ ap,
-                                                   const double* __restrict__ alpha, float* __restrict__ rn, size_t n,
-                                                   double* partial, unsigned* counter, const double* rr, double* rnrn,
-                                                   double* beta) {
-  const double a = *alpha;
-  double acc = 0.0;
-  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) {
-    const float v = (float)fma(-a, (double)ap[i], (double)r[i]);
-    rn[i] = v;
-    acc = fma((double)v, (double)v, acc);
-  }
-  acc = block_sum_f64(acc);
-  finish_sum(acc, partial, counter, [&](double tot) {
-    *rnrn = tot;
-    *beta = tot / *rr;
-  });
-}
-
 int grid_for(size_t n) { return reduce_grid(n); }
 
 pms_trace_event ev(int kind, int major, int a, int b, int c, double x, double y) {
@@ -173,6 +229,23 @@ pms_trace_event ev(int kind, int major, int a, int b, int c, double x, double y)
   e.x = x;
   e.y = y;
   return e;
+}
+
+int cg_grid(size_t n) {
+  const size_t b = ((n >> 2) + kT - 1) / kT;
+  return (int)(b < 1 ? 1 : (b > 1024 ? 1024 : b));
+}
+
+template <class K>
+void launch_cg(K kernel, const CgBatch& b, int nb, hipStream_t st) {
+  hipLaunchKernelGGL(kernel, dim3(cg_grid(b.n), nb), dim3(kT), 0, st, b);
+  PMS_LAUNCH_CHECK();
+}
+
+template <class K>
+void launch_cg_fin(K kernel, const CgBatch& b, int nb, hipStream_t st) {
+  hipLaunchKernelGGL(kernel, dim3(1, nb), dim3(kT), 0, st, b, cg_grid(b.n));
+  PMS_LAUNCH_CHECK();
 }
 
 }  // namespace
@@ -229,28 +302,6 @@ Geometry make_geometry(int n, int psf_n, const MoresaneParams& prm, int scale_co
   return g;
 }
 
-// --------------------------------------------------------------------------- job
-// Host waits on the job's stream, timed so callers can tell GPU-bound from host-bound runs.
-void Job::sync() {
-  const auto t0 = std::chrono::steady_clock::now();
-  PMS_HIP(hipStreamSynchronize(st_));
-  wait_seconds_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-}
-
-// A job handed the legacy NULL stream runs on a stream of its own (graph capture needs one); every
-// moresane()/moresane_by_scale() call synchronises that stream before returning.
-Job::Job(hipStream_t st) : st_(st) {
-  if (!st_) {
-    PMS_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
-    owns_stream_ = true;
-  }
-}
-Job::~Job() {
-  drop_graphs();
-  arena_.release();
-  by_scale_arena_.release();
-  if (owns_stream_) (void)hipStreamDestroy(st_);
-}
 
 // Byte sizes of every per-job buffer for geometry g (S = scale capacity).
 static JobSizes job_sizes(const Geometry& g, int S) {
@@ -306,14 +357,35 @@ size_t job_footprint_bytes(const Geometry& g) {
          z.partial + z.pmax + 64 * 8 + 64 * 4 + 12 * 256;
 }
 
+
+static std::vector<int> suppression_borders(int S, int edge_suppression, int edge_offset) {
+  std::vector<int> b;
+  if (edge_suppression) {
+    int corruption = 0;
+    for (int i = 0; i < S; ++i) {
+      corruption += 2 * (1 << i);
+      b.push_back(edge_offset > corruption ? edge_offset : corruption);
+    }
+  } else if (edge_offset > 0) {
+    b.assign(S, edge_offset);
+  }
+  return b;
+}
+
+
+// --------------------------------------------------------------------------- job workspace
+Job::~Job() {
+  arena_.release();
+  by_scale_arena_.release();
+}
+
 // Buffers grow to the largest geometry seen and are reused by smaller jobs.
-void Job::ensure(const Geometry& g) {
+bool Job::ensure(const Geometry& g, hipStream_t st) {
   const int S = std::max(g.S, (int)std::log2((double)g.n) - 1);
   const JobSizes need = job_sizes(g, S);
   geo_ = g;
-  if (have_ && fits(need, cap_)) return;
+  if (have_ && fits(need, cap_)) return false;
   const JobSizes z = have_ ? max_sizes(need, cap_) : need;
-  drop_graphs();
   arena_.release();
   cap_ = z;
   auto F = [&](size_t bytes) { return (float*)arena_.alloc(bytes); };
@@ -340,395 +412,645 @@ void Job::ensure(const Geometry& g) {
   fft_work = (float2*)arena_.alloc(z.fft);
   psf_frame = F(z.psf);
   mad_work = arena_.alloc(mad_work_bytes(16));
-  mad_work_init(mad_work, 16, st_);
+  mad_work_init(mad_work, 16, st);
   ext_work = arena_.alloc(z.ext);
   partial = (double*)arena_.alloc(z.partial);
   pmax = (float*)arena_.alloc(z.pmax);
-  scal = (double*)arena_.alloc(64 * sizeof(double));
-  iflag = (int*)arena_.alloc(64 * sizeof(int));
-  fill_u32((uint32_t*)iflag, 64, 0u, st_);  // flags and the grid-finish counters start at zero
-  border_dev = iflag + 16;
   have_ = true;
+  return true;
 }
 
-// psf_sub, PSF spectra and per-scale PSF energies (main.py:126-226)
-void Job::setup_psf(const float* psf_dev, std::vector<double>& energies) {
-  const Geometry& g = geo_;
-  spec_full = (g.p_full == g.p_sub && g.sub == g.n) ? spec_sub : spec_full_buf;
-  const int pn = g.psf_n;
-  const int plo = g.double_psf ? pn / 2 - g.sub / 2 : g.lo;
-  window_copy(psf_dev, pn, plo, plo, g.sub, nullptr, psf_sub, st_);
+float* Job::by_scale_dirty(size_t n2) {
+  if (n2 > by_scale_n2_) {
+    by_scale_arena_.release();
+    cur_dirty = (float*)by_scale_arena_.alloc(n2 * sizeof(float));
+    by_scale_n2_ = n2;
+  }
+  return cur_dirty;
+}
+
+// --------------------------------------------------------------------------- task
+Task::Task(const TaskSpec& s, Job* job, double* scal, int* iflag, double* upload_host)
+    : spec(s), job_(job), scal_(scal), iflag_(iflag), upload_(upload_host) {
+  if (spec.by_scale) {
+    trace.push_back(ev(EV_BYSCALE, 0, spec.start_scale, 0, 0, 0.0, 0.0));
+    start_call(spec.start_scale);
+  } else {
+    start_call(spec.prm.scale_count);
+  }
+}
+
+void Task::fail(int code, const std::string& msg) {
+  status = code;
+  error = msg;
+  phase_ = DONE;
+  iterate_ = false;
+}
+
+void Task::start_call(int scale_count) {
+  scale_count_ = scale_count;
+  phase_ = SETUP;
+}
+
+void Task::enqueue(hipStream_t st) {
+  if (upload_energies_ > 0) {
+    PMS_HIP(hipMemcpyAsync(scal_ + kEnergy, upload_, sizeof(double) * upload_energies_, hipMemcpyHostToDevice, st));
+    upload_energies_ = 0;
+  }
+  ++steps;
+  switch (phase_) {
+    case SETUP: enqueue_setup(st); break;
+    case DECOMP: enqueue_decomp(st); break;
+    case MINOR:
+      if (minor_first_) enqueue_minor_start(st);
+      break;
+    case UPDATE: enqueue_update(st, spec.prm.loop_gain); break;
+    case REVERT: enqueue_update(st, -spec.prm.loop_gain); break;
+    case FINISH: enqueue_finish(st); break;
+    case DONE: break;
+  }
+}
+
+// psf_sub, PSF spectra and per-scale PSF energies (main.py:126-226), the dirty subregion and its
+// maximum (main.py:229-262).
+void Task::enqueue_setup(hipStream_t st) {
+  const MoresaneParams& prm = spec.prm;
+  int sub = 0;
+  Geometry g = make_geometry(prm.n, prm.psf_n, prm, scale_count_, &sub);
+  if (job_->ensure(g, st) || !x_) {
+    moved_ = true;
+    x_ = job_->x;
+    xn_ = job_->xn;
+    r_ = job_->r;
+    rn_ = job_->rn;
+    rr_ = scal_ + kRR;
+    rnrn_ = scal_ + kRNRN;
+  }
+  g_ = job_->geometry();
+  const int S = g_.S;
+  PMS_REQUIRE(S <= 16, "scale_count above 16 is not supported");
+  const size_t N2 = (size_t)g_.n * g_.n;
+  const size_t s2 = (size_t)g_.sub * g_.sub;
+  if (spec.by_scale && !by_scale_copied_) {
+    by_scale_copied_ = true;
+    // the dirty image of each call is the previous residual; a private copy keeps the caller's intact
+    copy_plane(spec.dirty, job_->by_scale_dirty(N2), N2, nullptr, st);
+  }
+  Job& J = *job_;
+  J.spec_full = (g_.p_full == g_.p_sub && g_.sub == g_.n) ? J.spec_sub : J.spec_full_buf;
+  const float* psf_dev = spec.psf;
+  const int pn = g_.psf_n;
+  const int plo = g_.double_psf ? pn / 2 - g_.sub / 2 : g_.lo;
+  window_copy(psf_dev, pn, plo, plo, g_.sub, nullptr, J.psf_sub, st);
   // spectra: the frame the reference transforms, expressed as (source window, q) for fft_psf_spectrum
   auto spectrum_of = [&](int lo, int q, int p, float2* dst) {
     const float* src = psf_dev;
     if (!(lo == 0 && q == pn)) {
-      window_copy(psf_dev, pn, lo, lo, q, nullptr, psf_frame, st_);
-      src = psf_frame;
+      window_copy(psf_dev, pn, lo, lo, q, nullptr, J.psf_frame, st);
+      src = J.psf_frame;
     }
-    fft_psf_spectrum(src, q, p, dst, fft_work, st_);
+    fft_psf_spectrum(src, q, p, dst, J.fft_work, st);
   };
-  const bool full = g.sub == g.n;
-  if (g.linear) {
-    if (g.double_psf) {
-      if (full) {
-        spectrum_of(0, pn, g.p_full, spec_sub);
-      } else {
-        spectrum_of(pn / 2 - g.sub, 2 * g.sub, g.p_sub, spec_sub);
-        spectrum_of(0, pn, g.p_full, spec_full);
-      }
+  const bool full = g_.sub == g_.n;
+  if (g_.linear) {
+    // a 2N PSF is cropped to the frame; an N x N PSF is zero-padded to 2N (pad_array): q = N
+    if (full) {
+      spectrum_of(0, pn, g_.p_full, J.spec_sub);
     } else {
-      // N x N PSF zero-padded to 2N (pad_array): q = N inside a 2N frame
-      if (full) {
-        spectrum_of(0, pn, g.p_full, spec_sub);
-      } else {
-        spectrum_of(pn / 2 - g.sub, 2 * g.sub, g.p_sub, spec_sub);
-        spectrum_of(0, pn, g.p_full, spec_full);
-      }
+      spectrum_of(pn / 2 - g_.sub, 2 * g_.sub, g_.p_sub, J.spec_sub);
+      spectrum_of(0, pn, g_.p_full, J.spec_full);
     }
   } else {
-    if (g.double_psf) {
-      spectrum_of(pn / 2 - g.sub / 2, g.sub, g.p_sub, spec_sub);
-      if (spec_full != spec_sub) spectrum_of(pn / 2 - g.n / 2, g.n, g.p_full, spec_full);
+    if (g_.double_psf) {
+      spectrum_of(pn / 2 - g_.sub / 2, g_.sub, g_.p_sub, J.spec_sub);
+      if (J.spec_full != J.spec_sub) spectrum_of(pn / 2 - g_.n / 2, g_.n, g_.p_full, J.spec_full);
     } else {
-      spectrum_of(g.lo, g.sub, g.p_sub, spec_sub);
-      if (spec_full != spec_sub) spectrum_of(0, pn, g.p_full, spec_full);
+      spectrum_of(g_.lo, g_.sub, g_.p_sub, J.spec_sub);
+      if (J.spec_full != J.spec_sub) spectrum_of(0, pn, g_.p_full, J.spec_full);
     }
   }
-  // energies of the PSF decomposition
-  const size_t s2 = (size_t)g.sub * g.sub;
-  iuwt_decompose(psf_sub, g.sub, g.sub, g.S, 0, D, nullptr, iuwt_work, nullptr, nullptr, nullptr, nullptr, st_);
-  const int gr = reduce_grid(s2);
-  plane_sumsq_partials(D, g.S, s2, partial, nullptr, st_);
-  sum_partials(partial, gr, g.S, scal + 32, nullptr, st_);
-  std::vector<double> e2(g.S);
-  PMS_HIP(hipMemcpyAsync(e2.data(), scal + 32, sizeof(double) * g.S, hipMemcpyDeviceToHost, st_));
-  sync();
-  energies.resize(g.S);
-  for (int i = 0; i < g.S; ++i) energies[i] = std::sqrt(e2[i]);
-  PMS_HIP(hipMemcpyAsync(scal + 32, energies.data(), sizeof(double) * g.S, hipMemcpyHostToDevice, st_));
+  // squared energies of the PSF decomposition; their square roots come back as an upload
+  iuwt_decompose(J.psf_sub, g_.sub, g_.sub, S, 0, J.D, nullptr, J.iuwt_work, nullptr, nullptr, nullptr, nullptr, st);
+  plane_sumsq_partials(J.D, S, s2, J.partial, nullptr, st);
+  sum_partials(J.partial, reduce_grid(s2), S, scal_ + kEnergy, nullptr, st);
+  nm_ = noise_mask(g_.sub, g_.sub, prm.edge_excl, prm.int_excl);
+  borders_ = suppression_borders(S, prm.edge_suppression, prm.edge_offset);
+  if (!borders_.empty()) {
+    int* ub = (int*)(upload_ + 32);
+    std::memcpy(ub, borders_.data(), sizeof(int) * S);
+    PMS_HIP(hipMemcpyAsync(iflag_ + kBorders, ub, sizeof(int) * S, hipMemcpyHostToDevice, st));
+  }
+  // model (local to this call) starts at zero; dirty subregion window and its maximum
+  fill_f32(J.model_local, N2, 0.f, st);
+  window_copy(dirty_dev(), g_.n, g_.lo, g_.lo, g_.sub, nullptr, J.dirty_sub, st);
+  plane_max(J.dirty_sub, 1, s2, 0, J.pmax, nullptr, st);
+  max_final(J.pmax, threshold_grid(s2), 1, (float*)(scal_ + kDmax), nullptr, nullptr, nullptr, st);
 }
 
-// residual = dirty - H model (full image); returns std of residual[subregion]
-double Job::residual_and_std(const float* dirty_dev, const float* model_dev) {
-  const Geometry& g = geo_;
-  const size_t N2 = (size_t)g.n * g.n;
-  fft_conv(model_dev, g.n, g.p_full, spec_full, conv_full, fft_work, nullptr, st_);
-  hipLaunchKernelGGL(k_sub, dim3(grid_for(N2)), dim3(kT), 0, st_, dirty_dev, (const float*)conv_full, resid, N2);
+// decomposition, MAD thresholds, (mask), thresholding and normalised maxima (main.py:278-300)
+void Task::enqueue_decomp(hipStream_t st) {
+  const MoresaneParams& prm = spec.prm;
+  Job& J = *job_;
+  const int S = g_.S;
+  const int sub = g_.sub;
+  const size_t s2 = (size_t)sub * sub;
+  iuwt_decompose(J.dirty_sub, sub, sub, S, 0, J.D, nullptr, J.iuwt_work, nullptr, nullptr, nullptr, nullptr, st);
+  mad_thresholds(J.D, S, sub, sub, nm_, scal_ + kThr, J.mad_work, nullptr, st);
+  if (spec.mask) {
+    // decomposition of dirty_sub * mask[sub] replaces the thresholded one (main.py:283-285)
+    window_copy(spec.mask, g_.n, g_.lo, g_.lo, sub, nullptr, J.cbuf, st);
+    window_copy(J.dirty_sub, sub, 0, 0, sub, J.cbuf, J.rec, st);
+    iuwt_decompose(J.rec, sub, sub, S, 0, J.D, nullptr, J.iuwt_work, nullptr, nullptr, nullptr, nullptr, st);
+  }
+  apply_threshold(J.D, J.D, S, sub, sub, scal_ + kThr, prm.sigma_level,
+                  borders_.empty() ? nullptr : iflag_ + kBorders, J.pmax, nullptr, st);
+  max_final(J.pmax, threshold_grid(s2), S, nullptr, scal_ + kEnergy, scal_ + kNsm, nullptr, st);
+}
+
+// source extraction, recomposition and CG set-up (main.py:347-377)
+void Task::enqueue_minor_start(hipStream_t st) {
+  const MoresaneParams& prm = spec.prm;
+  Job& J = *job_;
+  const int sub = g_.sub;
+  const size_t s2 = (size_t)sub * sub;
+  const int adj = scale_adjust_;
+  const int ns = max_scale_ - adj;
+  const float* tslice = J.D + (size_t)adj * s2;
+  // masks/extracted live at planes [adj, max_scale) so the decomposition can index them by scale
+  uint8_t* Ms = J.M + (size_t)adj * s2;
+  float* Es = J.E + (size_t)adj * s2;
+  source_extract(tslice, ns, sub, sub, prm.tolerance, prm.neg_comp, Ms, Es, nullptr, J.ext_work, nullptr, st);
+  iuwt_recompose(Es, ns, sub, sub, adj, nullptr, J.rec, J.iuwt_work, nullptr, nullptr, st);
+  fill_f32(x_, s2, 0.f, st);
+  copy_plane(J.rec, r_, s2, nullptr, st);
+  copy_plane(J.rec, J.p, s2, nullptr, st);
+  dot_partials(r_, nullptr, s2, J.partial, nullptr, st);
+  sum_partials(J.partial, reduce_grid(s2), 1, rr_, nullptr, st);
+  plane_sumsq_partials(Es, 1, (size_t)ns * s2, J.partial, nullptr, st);
+  sum_partials(J.partial, reduce_grid((size_t)ns * s2), 1, scal_ + kE2, nullptr, st);
+}
+
+// model += gain x, residual = dirty - H model, std of residual[sub], max of residual[sub]
+// (main.py:479-506; a negative gain reverts the update)
+void Task::enqueue_update(hipStream_t st, double gain) {
+  Job& J = *job_;
+  const size_t N2 = (size_t)g_.n * g_.n;
+  const size_t s2 = (size_t)g_.sub * g_.sub;
+  hipLaunchKernelGGL(k_model_add, dim3(grid_for(s2)), dim3(kT), 0, st, J.model_local, g_.n, g_.lo, g_.lo, g_.sub,
+                     (const float*)x_, gain);
   PMS_LAUNCH_CHECK();
-  window_copy(resid, g.n, g.lo, g.lo, g.sub, nullptr, dirty_sub, st_);
-  const size_t s2 = (size_t)g.sub * g.sub;
-  mean_std(dirty_sub, s2, partial, scal + 56, scal + 57, st_);
-  double v[2];
-  PMS_HIP(hipMemcpyAsync(v, scal + 56, sizeof(double) * 2, hipMemcpyDeviceToHost, st_));
-  sync();
-  return std::sqrt(v[1] / (double)s2);
+  fft_conv(J.model_local, g_.n, g_.p_full, J.spec_full, J.conv_full, J.fft_work, nullptr, st);
+  hipLaunchKernelGGL(k_sub, dim3(grid_for(N2)), dim3(kT), 0, st, dirty_dev(), (const float*)J.conv_full, J.resid,
+                     N2);
+  PMS_LAUNCH_CHECK();
+  window_copy(J.resid, g_.n, g_.lo, g_.lo, g_.sub, nullptr, J.dirty_sub, st);
+  mean_std(J.dirty_sub, s2, J.partial, scal_ + kMeanSum, scal_ + kDev2, st);
+  plane_max(J.dirty_sub, 1, s2, 0, J.pmax, nullptr, st);
+  max_final(J.pmax, threshold_grid(s2), 1, (float*)(scal_ + kDmax), nullptr, nullptr, nullptr, st);
 }
 
-static std::vector<int> suppression_borders(int S, int edge_suppression, int edge_offset) {
-  std::vector<int> b;
-  if (edge_suppression) {
-    int corruption = 0;
-    for (int i = 0; i < S; ++i) {
-      corruption += 2 * (1 << i);
-      b.push_back(edge_offset > corruption ? edge_offset : corruption);
-    }
-  } else if (edge_offset > 0) {
-    b.assign(S, edge_offset);
+void Task::enqueue_finish(hipStream_t st) {
+  Job& J = *job_;
+  const size_t N2 = (size_t)g_.n * g_.n;
+  if (major_ > 0) {
+    hipLaunchKernelGGL(k_add_inplace, dim3(grid_for(N2)), dim3(kT), 0, st, spec.model, (const float*)J.model_local,
+                       N2);
+    PMS_LAUNCH_CHECK();
+    copy_plane(J.resid, spec.residual, N2, nullptr, st);
   }
+  if (spec.by_scale) copy_plane(spec.residual, J.cur_dirty, N2, nullptr, st);
+}
+
+IterArgs Task::iter_args() const {
+  return IterArgs{x_, xn_, r_, rn_, job_->p, job_->ap, rr_, rnrn_, max_scale_, scale_adjust_,
+                  spec.prm.enforce_positivity, g_.sub, g_.p_sub};
+}
+
+// ---- host decisions
+void Task::consume(const double* h) {
+  switch (phase_) {
+    case SETUP: {
+      for (int i = 0; i < g_.S; ++i) upload_[i] = std::sqrt(h[kEnergy + i]);
+      upload_energies_ = g_.S;
+      std::memcpy(&dmax_, h + kDmax, sizeof(float));
+      major_ = 0;
+      max_coeff_ = 1.0;
+      std_current_ = 1000.0;
+      std_ratio_ = 1.0;
+      min_scale_ = 0;
+      nsm_.assign(g_.S, 0.0);
+      major_check();
+      break;
+    }
+    case DECOMP:
+      for (int i = 0; i < g_.S; ++i) nsm_[i] = h[kNsm + i];
+      choose_scales();
+      break;
+    case MINOR: {
+      if (minor_first_) {
+        e2_ = h[kE2];
+        minor_first_ = false;
+      }
+      if (!iterate_) {
+        minor_exit();
+        break;
+      }
+      const double d2 = h[kD2];
+      snr_last_ = snr_current_;
+      snr_current_ = 20.0 * std::log10(std::sqrt(e2_) / std::sqrt(d2));
+      ++niter_;
+      if (niter_ == 1 && snr_current_ > 40) {
+        ++min_scale_;
+        exit_code_ = EXIT_FIRST_HIGH;
+        minor_exit();
+        break;
+      }
+      if (snr_current_ > 40) {
+        std::swap(x_, xn_);
+        min_scale_ = 0;
+        exit_code_ = EXIT_ACCEPT;
+        minor_exit();
+        break;
+      }
+      if (niter_ > 2 && snr_current_ <= snr_last_) {
+        if (snr_current_ > 10.5) {
+          min_scale_ = 0;
+          exit_code_ = EXIT_FELL_OK;
+        } else {
+          ++min_scale_;
+          exit_code_ = EXIT_FELL_LOW;
+        }
+        minor_exit();
+        break;
+      }
+      std::swap(r_, rn_);
+      std::swap(x_, xn_);
+      std::swap(rr_, rnrn_);
+      if (niter_ >= spec.prm.minor_loop_miter) {
+        exit_code_ = EXIT_CAP;
+        minor_exit();
+      }
+      break;
+    }
+    case UPDATE: {
+      const size_t s2 = (size_t)g_.sub * g_.sub;
+      const double std_last = std_current_;
+      std_current_ = std::sqrt(h[kDev2] / (double)s2);
+      std_ratio_ = (std_last - std_current_) / std_last;
+      if (std_ratio_ < 0) {
+        phase_ = REVERT;
+        break;
+      }
+      trace.push_back(ev(EV_UPDATE, major_, 0, 0, 0, std_current_, std_ratio_));
+      std::memcpy(&dmax_, h + kDmax, sizeof(float));
+      ++major_;
+      major_check();
+      break;
+    }
+    case REVERT:
+      trace.push_back(ev(EV_UPDATE, major_, 1, 0, 0, std_current_, std_ratio_));
+      std::memcpy(&dmax_, h + kDmax, sizeof(float));
+      ++major_;
+      major_check();
+      break;
+    case FINISH: {
+      major_iters = major_;
+      if (!spec.by_scale) {
+        phase_ = DONE;
+        break;
+      }
+      ++scale_count_;
+      if (complete || scale_count_ > std::log2((double)spec.prm.n) - 1 || scale_count_ > spec.stop_scale) {
+        phase_ = DONE;
+        break;
+      }
+      trace.push_back(ev(EV_BYSCALE, 0, scale_count_, 0, 0, 0.0, 0.0));
+      start_call(scale_count_);
+      break;
+    }
+    case DONE: break;
+  }
+}
+
+// top of the major loop (main.py:266)
+void Task::major_check() {
+  const MoresaneParams& prm = spec.prm;
+  if (major_ < prm.major_loop_miter && max_coeff_ > 0 && std_ratio_ > prm.accuracy &&
+      (double)dmax_ > prm.flux_threshold) {
+    max_scale_ = 0;
+    scale_adjust_ = 0;
+    inner_check();
+  } else {
+    phase_ = FINISH;
+  }
+}
+
+// top of the scale-selection loop (main.py:274)
+void Task::inner_check() {
+  if (min_scale_ < g_.S) {
+    if (min_scale_ == 0)
+      phase_ = DECOMP;
+    else
+      choose_scales();
+  } else {
+    after_inner();
+  }
+}
+
+// scale choice from the normalised maxima (main.py:304-340)
+void Task::choose_scales() {
+  const int S = g_.S;
+  int max_index = min_scale_;
+  for (int i = min_scale_ + 1; i < S; ++i)
+    if (nsm_[i] > nsm_[max_index]) max_index = i;
+  max_scale_ = max_index + 1;
+  max_coeff_ = nsm_[max_index];
+  if (max_coeff_ == 0) {
+    trace.push_back(ev(EV_NOCOEFF, major_, 0, 0, 0, 0.0, 0.0));
+    after_inner();
+    return;
+  }
+  scale_adjust_ = 0;
+  for (int i = max_index - 1; i >= 0; --i)
+    if (nsm_[i] == 0) {
+      scale_adjust_ = i + 1;
+      break;
+    }
+  trace.push_back(ev(EV_SCALES, major_, min_scale_, max_scale_, scale_adjust_, max_coeff_, 0.0));
+  minor_first_ = true;
+  iterate_ = spec.prm.minor_loop_miter > 0;
+  niter_ = 0;
+  exit_code_ = EXIT_CAP;
+  snr_last_ = 0.0;
+  snr_current_ = 0.0;
+  phase_ = MINOR;
+}
+
+// end of one minor loop (main.py:459-475)
+void Task::minor_exit() {
+  iterate_ = false;
+  trace.push_back(ev(EV_MINOR, major_, niter_, exit_code_, 0, snr_current_, 0.0));
+  if (niter_ == spec.prm.minor_loop_miter && snr_current_ > 10.5) {
+    min_scale_ = 0;
+    after_inner();
+    return;
+  }
+  if (min_scale_ == 0) {
+    after_inner();
+    return;
+  }
+  PMS_REQUIRE(!(exit_code_ == EXIT_CAP && min_scale_ > 0),
+              "minor loop hit its cap at SNR <= 10.5 with min_scale > 0: the reference loops forever here");
+  inner_check();
+}
+
+// after the scale-selection loop (main.py:477-516)
+void Task::after_inner() {
+  if (min_scale_ == g_.S) {
+    trace.push_back(ev(EV_POOR, major_, 0, 0, 0, 0.0, 0.0));
+    phase_ = FINISH;
+    return;
+  }
+  if (max_coeff_ > 0) {
+    phase_ = UPDATE;
+    return;
+  }
+  if (major_ == 0) {
+    trace.push_back(ev(EV_NOWORK, 0, 0, 0, 0, 0.0, 0.0));
+    complete = 1;
+    phase_ = FINISH;
+    return;
+  }
+  major_check();
+}
+
+// --------------------------------------------------------------------------- runner
+Runner::Runner(int slots, hipStream_t st) : st_(st) {
+  PMS_REQUIRE(slots >= 1 && slots <= 1024, "runner slots must be in 1..1024");
+  if (!st_) {
+    PMS_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    owns_stream_ = true;
+  }
+  jobs_.resize(slots);
+  tasks_.resize(slots);
+  for (auto& j : jobs_) j = std::make_unique<Job>();
+  PMS_HIP(hipMalloc(&scal_dev_, sizeof(double) * kSlotDoubles * slots));
+  PMS_HIP(hipMalloc(&iflag_dev_, sizeof(int) * kSlotInts * slots));
+  fill_u32((uint32_t*)scal_dev_, (size_t)2 * kSlotDoubles * slots, 0u, st_);
+  fill_u32((uint32_t*)iflag_dev_, (size_t)kSlotInts * slots, 0u, st_);  // flags and finish counters start at 0
+  PMS_HIP(hipHostMalloc((void**)&scal_host_, sizeof(double) * kSlotDoubles * slots, hipHostMallocDefault));
+  PMS_HIP(hipHostMalloc((void**)&upload_host_, sizeof(double) * kSlotDoubles * slots, hipHostMallocDefault));
+  const char* graph_env = getenv("PMS_CG_GRAPH");
+  use_graph_ = graph_env && graph_env[0] == '1';
+  PMS_HIP(hipStreamSynchronize(st_));
+}
+
+Runner::~Runner() {
+  if (st_) (void)hipStreamSynchronize(st_);
+  drop_graphs();
+  tasks_.clear();
+  jobs_.clear();
+  if (scal_dev_) (void)hipFree(scal_dev_);
+  if (iflag_dev_) (void)hipFree(iflag_dev_);
+  if (scal_host_) (void)hipHostFree(scal_host_);
+  if (upload_host_) (void)hipHostFree(upload_host_);
+  if (owns_stream_) (void)hipStreamDestroy(st_);
+}
+
+int Runner::free_slots() const {
+  int n = 0;
+  for (auto& t : tasks_)
+    if (!t) ++n;
+  return n;
+}
+
+bool Runner::idle() const {
+  for (auto& t : tasks_)
+    if (t && !t->done()) return false;
+  return true;
+}
+
+size_t Runner::device_bytes() const {
+  size_t b = 0;
+  for (auto& j : jobs_) b += j->device_bytes();
   return b;
 }
 
-int Job::moresane(const MoresaneParams& prm, int scale_count, const float* dirty_dev, const float* psf_dev,
-                  const float* mask_dev, float* model_dev, float* residual_dev, int* complete, Trace& trace) {
-  int sub = 0;
-  Geometry g = make_geometry(prm.n, prm.psf_n, prm, scale_count, &sub);
-  ensure(g);
-  g = geo_;
-  const int S = g.S;
-  const size_t s2 = (size_t)g.sub * g.sub;
-  const size_t N2 = (size_t)g.n * g.n;
-  const int gr = reduce_grid(s2);
-  std::vector<double> energies;
-  setup_psf(psf_dev, energies);
-  NoiseMask nm = noise_mask(g.sub, g.sub, prm.edge_excl, prm.int_excl);
-  std::vector<int> borders = suppression_borders(S, prm.edge_suppression, prm.edge_offset);
-  if (!borders.empty())
-    PMS_HIP(hipMemcpyAsync(border_dev, borders.data(), sizeof(int) * S, hipMemcpyHostToDevice, st_));
-
-  // model (local to this call) starts at zero; dirty subregion window
-  fill_f32(model_local, N2, 0.f, st_);
-  window_copy(dirty_dev, g.n, g.lo, g.lo, g.sub, nullptr, dirty_sub, st_);
-  plane_max(dirty_sub, 1, s2, 0, pmax, nullptr, st_);
-  max_final(pmax, threshold_grid(s2), 1, (float*)(scal + 60), nullptr, nullptr, nullptr, st_);
-  float dmax = 0.f;
-  PMS_HIP(hipMemcpyAsync(&dmax, scal + 60, sizeof(float), hipMemcpyDeviceToHost, st_));
-  sync();
-
-  double* thr = scal + 0;       // [0, 16)  thresholds (S <= 16)
-  double* nsm_dev = scal + 16;  // [16, 32) normalised maxima
-  double* energy_dev = scal + 32;
-  double* d_rr = scal + 48;
-  double* d_rnrn = scal + 51;
-  double* d_d2 = scal + 53;
-  double* d_e2 = scal + 54;
-  PMS_REQUIRE(S <= 16, "scale_count above 16 is not supported");
-
-  // Replaying the iteration from a captured HIP graph is bit-identical to eager launches but measured
-  // slower on ROCm 7 for large frames, so it is opt-in (PMS_CG_GRAPH=1).
-  const char* graph_env = getenv("PMS_CG_GRAPH");
-  const bool use_graph = graph_env && graph_env[0] == '1';
-  int major = 0;
-  double max_coeff = 1.0;
-  double std_current = 1000.0, std_ratio = 1.0;
-  int min_scale = 0;
-  std::vector<double> nsm(S, 0.0);
-  const int pmx = threshold_grid(s2);
-
-  while (major < prm.major_loop_miter && max_coeff > 0 && std_ratio > prm.accuracy &&
-         (double)dmax > prm.flux_threshold) {
-    int max_scale = 0, scale_adjust = 0;
-    while (min_scale < S) {
-      if (min_scale == 0) {
-        iuwt_decompose(dirty_sub, g.sub, g.sub, S, 0, D, nullptr, iuwt_work, nullptr, nullptr, nullptr, nullptr, st_);
-        mad_thresholds(D, S, g.sub, g.sub, nm, thr, mad_work, nullptr, st_);
-        if (mask_dev) {
-          // decomposition of dirty_sub * mask[sub] replaces the thresholded one (main.py:283-285)
-          window_copy(mask_dev, g.n, g.lo, g.lo, g.sub, nullptr, cbuf, st_);
-          window_copy(dirty_sub, g.sub, 0, 0, g.sub, cbuf, rec, st_);
-          iuwt_decompose(rec, g.sub, g.sub, S, 0, D, nullptr, iuwt_work, nullptr, nullptr, nullptr, nullptr, st_);
-        }
-        apply_threshold(D, D, S, g.sub, g.sub, thr, prm.sigma_level, borders.empty() ? nullptr : border_dev, pmax,
-                        nullptr, st_);
-        max_final(pmax, pmx, S, nullptr, energy_dev, nsm_dev, nullptr, st_);
-        PMS_HIP(hipMemcpyAsync(nsm.data(), nsm_dev, sizeof(double) * S, hipMemcpyDeviceToHost, st_));
-        sync();
-      }
-      int max_index = min_scale;
-      for (int i = min_scale + 1; i < S; ++i)
-        if (nsm[i] > nsm[max_index]) max_index = i;
-      max_scale = max_index + 1;
-      max_coeff = nsm[max_index];
-      if (max_coeff == 0) {
-        trace.push_back(ev(EV_NOCOEFF, major, 0, 0, 0, 0.0, 0.0));
-        break;
-      }
-      scale_adjust = 0;
-      for (int i = max_index - 1; i >= 0; --i)
-        if (nsm[i] == 0) {
-          scale_adjust = i + 1;
-          break;
-        }
-      trace.push_back(ev(EV_SCALES, major, min_scale, max_scale, scale_adjust, max_coeff, 0.0));
-      const int ns = max_scale - scale_adjust;
-      const float* tslice = D + (size_t)scale_adjust * s2;
-      // masks/extracted live at planes [adj, max_scale) so the decomposition can index them by scale
-      uint8_t* Ms = M + (size_t)scale_adjust * s2;
-      float* Es = E + (size_t)scale_adjust * s2;
-      source_extract(tslice, ns, g.sub, g.sub, prm.tolerance, prm.neg_comp, Ms, Es, nullptr, ext_work, nullptr, st_);
-      iuwt_recompose(Es, ns, g.sub, g.sub, scale_adjust, nullptr, rec, iuwt_work, nullptr, nullptr, st_);
-
-      // ---- CG minor loop
-      fill_f32(x, s2, 0.f, st_);
-      copy_plane(rec, r, s2, nullptr, st_);
-      copy_plane(rec, p, s2, nullptr, st_);
-      dot_partials(r, nullptr, s2, partial, nullptr, st_);
-      sum_partials(partial, gr, 1, d_rr, nullptr, st_);
-      plane_sumsq_partials(Es, 1, (size_t)ns * s2, partial, nullptr, st_);
-      sum_partials(partial, reduce_grid((size_t)ns * s2), 1, d_e2, nullptr, st_);
-      double e2 = 0.0;
-      PMS_HIP(hipMemcpyAsync(&e2, d_e2, sizeof(double), hipMemcpyDeviceToHost, st_));
-
-      int niter = 0;
-      double snr_last = 0.0, snr_current = 0.0;
-      int exit_code = EXIT_CAP;
-      while (niter < prm.minor_loop_miter) {
-        IterArgs ia{x, xn, r, rn, p, ap, d_rr, d_rnrn, max_scale, scale_adjust, prm.enforce_positivity, g.sub,
-                    g.p_sub};
-        run_iteration(ia, use_graph);
-        double d2 = 0.0;
-        PMS_HIP(hipMemcpyAsync(&d2, d_d2, sizeof(double), hipMemcpyDeviceToHost, st_));
-        sync();
-        snr_last = snr_current;
-        snr_current = 20.0 * std::log10(std::sqrt(e2) / std::sqrt(d2));
-        ++niter;
-        if (niter == 1 && snr_current > 40) {
-          ++min_scale;
-          exit_code = EXIT_FIRST_HIGH;
-          break;
-        }
-        if (snr_current > 40) {
-          std::swap(x, xn);
-          min_scale = 0;
-          exit_code = EXIT_ACCEPT;
-          break;
-        }
-        if (niter > 2 && snr_current <= snr_last) {
-          if (snr_current > 10.5) {
-            min_scale = 0;
-            exit_code = EXIT_FELL_OK;
-          } else {
-            ++min_scale;
-            exit_code = EXIT_FELL_LOW;
-          }
-          break;
-        }
-        std::swap(r, rn);
-        std::swap(x, xn);
-        std::swap(d_rr, d_rnrn);
-      }
-      trace.push_back(ev(EV_MINOR, major, niter, exit_code, 0, snr_current, 0.0));
-      if (niter == prm.minor_loop_miter && snr_current > 10.5) {
-        min_scale = 0;
-        break;
-      }
-      if (min_scale == 0) break;
-      PMS_REQUIRE(!(exit_code == EXIT_CAP && min_scale > 0),
-                  "minor loop hit its cap at SNR <= 10.5 with min_scale > 0: the reference loops forever here");
-    }
-
-    if (min_scale == S) {
-      trace.push_back(ev(EV_POOR, major, 0, 0, 0, 0.0, 0.0));
-      break;
-    }
-    if (max_coeff > 0) {
-      hipLaunchKernelGGL(k_model_add, dim3(grid_for(s2)), dim3(kT), 0, st_, model_local, g.n, g.lo, g.lo, g.sub,
-                         (const float*)x, prm.loop_gain);
-      PMS_LAUNCH_CHECK();
-      const double std_last = std_current;
-      std_current = residual_and_std(dirty_dev, model_local);
-      std_ratio = (std_last - std_current) / std_last;
-      const bool reverted = std_ratio < 0;
-      if (reverted) {
-        hipLaunchKernelGGL(k_model_add, dim3(grid_for(s2)), dim3(kT), 0, st_, model_local, g.n, g.lo, g.lo, g.sub,
-                           (const float*)x, -prm.loop_gain);
-        PMS_LAUNCH_CHECK();
-        residual_and_std(dirty_dev, model_local);
-      }
-      trace.push_back(ev(EV_UPDATE, major, reverted ? 1 : 0, 0, 0, std_current, std_ratio));
-      // dirty_sub already holds residual[sub] (written by residual_and_std)
-      plane_max(dirty_sub, 1, s2, 0, pmax, nullptr, st_);
-      max_final(pmax, pmx, 1, (float*)(scal + 60), nullptr, nullptr, nullptr, st_);
-      PMS_HIP(hipMemcpyAsync(&dmax, scal + 60, sizeof(float), hipMemcpyDeviceToHost, st_));
-      sync();
-      ++major;
-    }
-    if (major == 0) {
-      trace.push_back(ev(EV_NOWORK, 0, 0, 0, 0, 0.0, 0.0));
-      *complete = 1;
-      break;
-    }
-  }
-  if (major > 0) {
-    hipLaunchKernelGGL(k_add_inplace, dim3(grid_for(N2)), dim3(kT), 0, st_, model_dev, (const float*)model_local, N2);
-    PMS_LAUNCH_CHECK();
-    copy_plane(resid, residual_dev, N2, nullptr, st_);
-  }
-  sync();
-  return major;
+int Runner::first_free() const {
+  for (int k = 0; k < slots(); ++k)
+    if (!tasks_[k]) return k;
+  return -1;
 }
 
-// One CG iteration (main.py:380-424) up to the SNR sums, enqueued on the job's stream.
-void Job::enqueue_iteration(const IterArgs& a) {
-  const Geometry& g = geo_;
-  const size_t s2 = (size_t)g.sub * g.sub;
-  const int gr = reduce_grid(s2);
-  double* d_pap = scal + 49;
-  double* d_alpha = scal + 50;
-  double* d_beta = scal + 52;
-  double* d_d2 = scal + 53;
-  int* clip = iflag + 0;
-  unsigned* ctr = (unsigned*)(iflag + 40);
-  uint8_t* Ms = M + (size_t)a.adj * s2;
-  const float* Es = E + (size_t)a.adj * s2;
-  apply_op_masked(a.p, a.ap, a.max_scale, a.adj);
-  hipLaunchKernelGGL(k_cg_alpha, dim3(gr), dim3(kT), 0, st_, (const float*)a.p, (const float*)a.ap, s2, partial,
-                     ctr, (const double*)a.rr, d_pap, d_alpha);
-  PMS_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_cg_xn_flag, dim3(gr), dim3(kT), 0, st_, (const float*)a.x, (const float*)a.p,
-                     (const double*)d_alpha, a.xn, s2, partial, ctr + 1, a.enforce, clip);
-  PMS_LAUNCH_CHECK();
-  if (a.enforce) {
-    hipLaunchKernelGGL(k_cg_clip, dim3(gr), dim3(kT), 0, st_, a.xn, (const float*)a.x, a.p, (const double*)d_alpha,
-                       s2, (const int*)clip);
-    PMS_LAUNCH_CHECK();
-    apply_op_masked(a.p, a.ap, a.max_scale, a.adj, clip);
-  }
-  hipLaunchKernelGGL(k_cg_rn_beta, dim3(gr), dim3(kT), 0, st_, (const float*)a.r, (const float*)a.ap,
-                     (const double*)d_alpha, a.rn, s2, partial, ctr + 2, (const double*)a.rr, a.rnrn, d_beta);
-  PMS_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_cg_p, dim3(gr), dim3(kT), 0, st_, (const float*)a.rn, a.p, (const double*)d_beta, s2);
-  PMS_LAUNCH_CHECK();
-  // model sources and the SNR sums (main.py:413-424)
-  fft_conv(a.xn, g.sub, g.p_sub, spec_sub, cbuf, fft_work, nullptr, st_);
-  const long np = iuwt_decompose(cbuf, g.sub, g.sub, a.max_scale, a.adj, nullptr, nullptr, iuwt_work, Ms, Es,
-                                 partial, nullptr, st_);
-  sum_partials(partial, np, 1, d_d2, nullptr, st_);
+int Runner::add(const TaskSpec& spec, int slot) {
+  const int k = slot >= 0 ? slot : first_free();
+  if (k < 0) return -1;
+  PMS_REQUIRE(k < slots() && !tasks_[k], "runner slot is not free");
+  tasks_[k] = std::make_unique<Task>(spec, jobs_[k].get(), scal_dev_ + (size_t)k * kSlotDoubles,
+                                     iflag_dev_ + (size_t)k * kSlotInts, upload_host_ + (size_t)k * kSlotDoubles);
+  return k;
 }
 
-// Run one iteration, replaying a captured HIP graph for this exact set of buffers and scales when
-// enabled (the first iteration of each configuration records it).
-void Job::run_iteration(const IterArgs& a, bool use_graph) {
-  if (!use_graph) {
-    enqueue_iteration(a);
-    return;
-  }
-  auto it = graphs_.find(a);
-  if (it == graphs_.end()) {
-    PMS_HIP(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+void Runner::release(int slot) { tasks_[slot].reset(); }
+
+void Runner::run() {
+  while (!idle()) step();
+}
+
+void Runner::step() {
+  std::vector<int> active;
+  for (int k = 0; k < slots(); ++k)
+    if (tasks_[k] && !tasks_[k]->done()) active.push_back(k);
+  if (active.empty()) return;
+  bool moved = false;
+  for (int k : active) {
+    Task& t = *tasks_[k];
     try {
-      enqueue_iteration(a);
-    } catch (...) {
-      hipGraph_t junk = nullptr;
-      (void)hipStreamEndCapture(st_, &junk);
-      if (junk) (void)hipGraphDestroy(junk);
-      throw;
+      t.enqueue(st_);
+    } catch (const Error& e) {
+      t.fail(e.code, e.msg);
     }
-    hipGraph_t graph = nullptr;
-    PMS_HIP(hipStreamEndCapture(st_, &graph));
-    hipGraphExec_t exec = nullptr;
-    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    PMS_HIP(e);
-    it = graphs_.emplace(a, exec).first;
+    moved = moved || t.geometry_moved();
   }
-  PMS_HIP(hipGraphLaunch(it->second, st_));
+  if (moved) drop_graphs();
+  std::vector<Task*> its;
+  for (int k : active)
+    if (tasks_[k]->wants_iteration()) its.push_back(tasks_[k].get());
+  if (!its.empty()) run_iterations(its);
+  PMS_HIP(hipMemcpyAsync(scal_host_, scal_dev_, sizeof(double) * kSlotDoubles * slots(), hipMemcpyDeviceToHost, st_));
+  const auto t0 = std::chrono::steady_clock::now();
+  PMS_HIP(hipStreamSynchronize(st_));
+  wait_seconds_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (int k : active) {
+    Task& t = *tasks_[k];
+    if (t.done()) continue;
+    try {
+      t.consume(scal_host_ + (size_t)k * kSlotDoubles);
+    } catch (const Error& e) {
+      t.fail(e.code, e.msg);
+    }
+  }
 }
 
-void Job::drop_graphs() {
+// Iterations of tasks that share a plane shape and FFT frame run together, kMaxBatch per launch.
+void Runner::run_iterations(std::vector<Task*>& ts) {
+  std::map<std::pair<int, int>, std::vector<Task*>> parts;
+  for (Task* t : ts) {
+    const IterArgs a = t->iter_args();
+    parts[{a.sub, a.p_sub}].push_back(t);
+  }
+  for (auto& kv : parts) {
+    std::vector<Task*>& v = kv.second;
+    for (size_t j0 = 0; j0 < v.size(); j0 += kMaxBatch) {
+      const size_t nb = std::min(v.size() - j0, (size_t)kMaxBatch);
+      std::vector<Task*> chunk(v.begin() + j0, v.begin() + j0 + nb);
+      std::vector<IterArgs> its;
+      for (Task* t : chunk) its.push_back(t->iter_args());
+      if (!use_graph_) {
+        enqueue_iterations(its, chunk);
+        continue;
+      }
+      // replay a captured graph of this exact set of buffers and scales (recorded on first use)
+      auto it = graphs_.find(its);
+      if (it == graphs_.end()) {
+        PMS_HIP(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+        try {
+          enqueue_iterations(its, chunk);
+        } catch (...) {
+          hipGraph_t junk = nullptr;
+          (void)hipStreamEndCapture(st_, &junk);
+          if (junk) (void)hipGraphDestroy(junk);
+          throw;
+        }
+        hipGraph_t graph = nullptr;
+        PMS_HIP(hipStreamEndCapture(st_, &graph));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        PMS_HIP(e);
+        it = graphs_.emplace(its, exec).first;
+      }
+      PMS_HIP(hipGraphLaunch(it->second, st_));
+    }
+  }
+}
+
+void Runner::drop_graphs() {
   for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
   graphs_.clear();
 }
 
-// Ap <- R(M . W(H v)) over scales [adj, max_scale); masks are indexed by scale
-void Job::apply_op_masked(const float* v, float* out, int max_scale, int adj, const int* pred) {
-  const Geometry& g = geo_;
-  fft_conv(v, g.sub, g.p_sub, spec_sub, cbuf, fft_work, pred, st_);
-  iuwt_decompose(cbuf, g.sub, g.sub, max_scale, adj, Dcg, nullptr, iuwt_work, M + (size_t)adj * g.sub * g.sub,
-                 nullptr, nullptr, pred, st_);
-  iuwt_recompose(Dcg, max_scale - adj, g.sub, g.sub, adj, nullptr, out, iuwt_work + 0, nullptr, pred, st_);
+// Ap <- R(M . W(H v)) over scales [adj, max_scale) for every task (main.py:380-390); the clip pass
+// recomputes it for tasks whose positivity clip fired (predicated on the device flag).
+void Runner::apply_op(const std::vector<Task*>& ts, const std::vector<IterArgs>& its, bool clip_pass) {
+  const int sub = its[0].sub, p_sub = its[0].p_sub;
+  const size_t s2 = (size_t)sub * sub;
+  std::vector<FftConvJob> fj;
+  std::vector<IuwtDecompJob> dj;
+  std::vector<IuwtRecompJob> rj;
+  for (size_t i = 0; i < ts.size(); ++i) {
+    Job& J = *ts[i]->job();
+    const IterArgs& a = its[i];
+    const int* pred = clip_pass ? ts[i]->iflag() + kClip : nullptr;
+    fj.push_back(FftConvJob{a.p, J.spec_sub, J.cbuf, J.fft_work, pred});
+    dj.push_back(IuwtDecompJob{J.cbuf, a.max_scale, a.adj, J.Dcg, nullptr, J.iuwt_work, J.M + (size_t)a.adj * s2,
+                               nullptr, nullptr, pred, 0});
+    rj.push_back(IuwtRecompJob{J.Dcg, a.max_scale - a.adj, a.adj, nullptr, a.ap, J.iuwt_work, nullptr, pred});
+  }
+  fft_conv_batch(fj.data(), (int)fj.size(), sub, p_sub, st_);
+  iuwt_decompose_batch(dj.data(), (int)dj.size(), sub, sub, st_);
+  iuwt_recompose_batch(rj.data(), (int)rj.size(), sub, sub, st_);
 }
 
-// main.py:523-599
-void Job::moresane_by_scale(const MoresaneParams& prm, int start_scale, int stop_scale, const float* dirty_dev,
-                            const float* psf_dev, const float* mask_dev, float* model_dev, float* residual_dev,
-                            Trace& trace) {
-  const int n = prm.n;
-  const size_t N2 = (size_t)n * n;
-  // the dirty image of each call is the previous residual; keep a private copy so the caller's dirty stays intact
-  float* cur_dirty = (float*)by_scale_arena_.alloc(N2 * sizeof(float));
-  copy_plane(dirty_dev, cur_dirty, N2, nullptr, st_);
-  int complete = 0;
-  int scale_count = start_scale;
-  while (!complete) {
-    trace.push_back(ev(EV_BYSCALE, 0, scale_count, 0, 0, 0.0, 0.0));
-    moresane(prm, scale_count, cur_dirty, psf_dev, mask_dev, model_dev, residual_dev, &complete, trace);
-    copy_plane(residual_dev, cur_dirty, N2, nullptr, st_);
-    ++scale_count;
-    if (scale_count > std::log2((double)n) - 1) break;
-    if (scale_count > stop_scale) break;
+// One CG iteration (main.py:380-424) up to the SNR sums for up to kMaxBatch tasks.
+void Runner::enqueue_iterations(const std::vector<IterArgs>& its, const std::vector<Task*>& ts) {
+  const int nb = (int)ts.size();
+  const int sub = its[0].sub, p_sub = its[0].p_sub;
+  const size_t s2 = (size_t)sub * sub;
+  CgBatch cb{};
+  cb.n = s2;
+  std::vector<Task*> clip_ts;
+  std::vector<IterArgs> clip_its;
+  CgBatch clip_b{};
+  clip_b.n = s2;
+  for (int i = 0; i < nb; ++i) {
+    const IterArgs& a = its[i];
+    Job& J = *ts[i]->job();
+    cb.j[i] = CgJob{a.x, a.xn, a.r, a.rn, a.p, a.ap, J.partial, ts[i]->scal(), a.rr, a.rnrn, ts[i]->iflag(),
+                    a.enforce};
+    if (a.enforce) {
+      clip_b.j[clip_ts.size()] = cb.j[i];
+      clip_ts.push_back(ts[i]);
+      clip_its.push_back(a);
+    }
   }
-  sync();
-  by_scale_arena_.release();
+  apply_op(ts, its, false);
+  launch_cg(k_cg_pap, cb, nb, st_);
+  launch_cg_fin(k_cg_alpha_fin, cb, nb, st_);
+  launch_cg(k_cg_xn, cb, nb, st_);
+  if (!clip_ts.empty()) {
+    launch_cg_fin(k_cg_flag_fin, clip_b, (int)clip_ts.size(), st_);
+    launch_cg(k_cg_clip, clip_b, (int)clip_ts.size(), st_);
+    apply_op(clip_ts, clip_its, true);
+  }
+  launch_cg(k_cg_rn, cb, nb, st_);
+  launch_cg_fin(k_cg_beta_fin, cb, nb, st_);
+  launch_cg(k_cg_p, cb, nb, st_);
+  // model sources and the SNR sums (main.py:413-424)
+  std::vector<FftConvJob> fj;
+  std::vector<IuwtDecompJob> dj;
+  for (int i = 0; i < nb; ++i) {
+    Job& J = *ts[i]->job();
+    const IterArgs& a = its[i];
+    fj.push_back(FftConvJob{a.xn, J.spec_sub, J.cbuf, J.fft_work, nullptr});
+    dj.push_back(IuwtDecompJob{J.cbuf, a.max_scale, a.adj, nullptr, nullptr, J.iuwt_work, J.M + (size_t)a.adj * s2,
+                               J.E + (size_t)a.adj * s2, J.partial, nullptr, 0});
+  }
+  fft_conv_batch(fj.data(), nb, sub, p_sub, st_);
+  iuwt_decompose_batch(dj.data(), nb, sub, sub, st_);
+  std::vector<SumJob> sj;
+  for (int i = 0; i < nb; ++i) sj.push_back(SumJob{ts[i]->job()->partial, dj[i].np, ts[i]->scal() + kD2});
+  sum_partials_batch(sj.data(), nb, st_);
 }
 
 }  // namespace pms

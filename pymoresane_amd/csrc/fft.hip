@@ -22,6 +22,7 @@
 // Twiddles come from a device table exp(-2 pi i k / 2^14) computed on the
 // host in double precision; in-butterfly powers are formed by a balanced
 // product tree from one table value.
+#include <algorithm>
 #include <cmath>
 #include <mutex>
 #include <vector>
@@ -254,13 +255,34 @@ __device__ __forceinline__ void mid_inverse(float2* sm, int L, int ns, int i0, i
   }
 }
 
+// Per-job pointers of a batched launch (grid y selects the job).
+struct R2cBatch {
+  const float* src[kMaxBatch];
+  float2* T[kMaxBatch];
+  const int* pred[kMaxBatch];
+};
+struct ColBatch {
+  const float2* T1[kMaxBatch];
+  const float2* S[kMaxBatch];
+  float2* T2[kMaxBatch];
+  const int* pred[kMaxBatch];
+};
+struct C2rBatch {
+  const float2* T2[kMaxBatch];
+  float* out[kMaxBatch];
+  const int* pred[kMaxBatch];
+};
+
 // ---- pass 1: rows r2c.  Group g of the block handles frame row fr; output T[k*ldT + fr], k in [0, P/2].
 // The frame row is the image row (off == 0, q == n) or the centred/rolled PSF (off = q/2).
 template <int R0>
-__global__ __launch_bounds__(kThreads) void k_rows_r2c(const float* __restrict__ src, int q, int off, int pm,
-                                                       int n_rows, int rb, float2* __restrict__ T, int ldT,
-                                                       const float2* __restrict__ tw, const int* pred) {
+__global__ __launch_bounds__(kThreads) void k_rows_r2c(R2cBatch bt, int q, int off, int pm, int n_rows, int rb,
+                                                       int ldT, const float2* __restrict__ tw) {
+  const int jb = blockIdx.y;
+  const int* pred = bt.pred[jb];
   if (pred && *pred == 0) return;
+  const float* __restrict__ src = bt.src[jb];
+  float2* __restrict__ T = bt.T[jb];
   extern __shared__ float2 smem[];
   const int P = 1 << pm;
   const int L = P >> 1;
@@ -308,11 +330,14 @@ __global__ __launch_bounds__(kThreads) void k_rows_r2c(const float* __restrict__
 // last forward stage + PSF product + first inverse stage fused in registers, inverse FFT with
 // its last stage fused with the store of the n_out kept rows.
 template <int R0>
-__global__ __launch_bounds__(kThreads) void k_cols_conv(const float2* __restrict__ T1, int n_in, int pm,
-                                                        const float2* __restrict__ S, float2* __restrict__ T2,
-                                                        int n_out, int cpb, const float2* __restrict__ tw,
-                                                        const int* pred) {
+__global__ __launch_bounds__(kThreads) void k_cols_conv(ColBatch bt, int n_in, int pm, int n_out, int cpb,
+                                                        const float2* __restrict__ tw) {
+  const int jb = blockIdx.y;
+  const int* pred = bt.pred[jb];
   if (pred && *pred == 0) return;
+  const float2* __restrict__ T1 = bt.T1[jb];
+  const float2* __restrict__ S = bt.S[jb];
+  float2* __restrict__ T2 = bt.T2[jb];
   extern __shared__ float2 smem[];
   const int P = 1 << pm;
   const int ncols = (P >> 1) + 1;
@@ -382,10 +407,13 @@ __global__ __launch_bounds__(kThreads) void k_cols_fwd(const float2* __restrict_
 
 // ---- pass 3: rows c2r.  Group g handles kept row r; out[r * n_cols + c], c < n_cols.
 template <int R0>
-__global__ __launch_bounds__(kThreads) void k_rows_c2r(const float2* __restrict__ T2, int ldT, int pm, int n_rows,
-                                                       int rb, float* __restrict__ out, int n_cols, float scale,
-                                                       const float2* __restrict__ tw, const int* pred) {
+__global__ __launch_bounds__(kThreads) void k_rows_c2r(C2rBatch bt, int ldT, int pm, int n_rows, int rb, int n_cols,
+                                                       float scale, const float2* __restrict__ tw) {
+  const int jb = blockIdx.y;
+  const int* pred = bt.pred[jb];
   if (pred && *pred == 0) return;
+  const float2* __restrict__ T2 = bt.T2[jb];
+  float* __restrict__ out = bt.out[jb];
   extern __shared__ float2 smem[];
   const int P = 1 << pm;
   const int L = P >> 1;
@@ -490,23 +518,23 @@ int first_radix(int m) { return 1 << plan_first_log2(m); }
     default: hipLaunchKernelGGL(KERNEL<16>, __VA_ARGS__); break;                  \
   }
 
-void launch_rows_r2c(const float* src, int q, int off, int pm, int n_rows, float2* T, int ldT, const float2* tw,
-                     const int* pred, hipStream_t st) {
+void launch_rows_r2c(const R2cBatch& bt, int count, int q, int off, int pm, int n_rows, int ldT, const float2* tw,
+                     hipStream_t st) {
   const int L = (1 << pm) / 2;
   const int rb = rows_per_block(L, n_rows);
   const size_t lds = (size_t)rb * (L + (L >> 5)) * sizeof(float2);
-  PMS_FFT_DISPATCH(first_radix(pm - 1), k_rows_r2c, dim3(ceil_div(n_rows, rb)), dim3(kThreads), lds, st, src, q, off,
-                   pm, n_rows, rb, T, ldT, tw, pred);
+  PMS_FFT_DISPATCH(first_radix(pm - 1), k_rows_r2c, dim3(ceil_div(n_rows, rb), count), dim3(kThreads), lds, st, bt,
+                   q, off, pm, n_rows, rb, ldT, tw);
   PMS_LAUNCH_CHECK();
 }
 
-void launch_rows_c2r(const float2* T2, int ldT, int pm, int n_rows, float* out, int n_cols, float scale,
-                     const float2* tw, const int* pred, hipStream_t st) {
+void launch_rows_c2r(const C2rBatch& bt, int count, int ldT, int pm, int n_rows, int n_cols, float scale,
+                     const float2* tw, hipStream_t st) {
   const int L = (1 << pm) / 2;
   const int rb = rows_per_block(L, n_rows);
   const size_t lds = (size_t)rb * (L + (L >> 5)) * sizeof(float2);
-  PMS_FFT_DISPATCH(first_radix(pm - 1), k_rows_c2r, dim3(ceil_div(n_rows, rb)), dim3(kThreads), lds, st, T2, ldT,
-                   pm, n_rows, rb, out, n_cols, scale, tw, pred);
+  PMS_FFT_DISPATCH(first_radix(pm - 1), k_rows_c2r, dim3(ceil_div(n_rows, rb), count), dim3(kThreads), lds, st, bt,
+                   ldT, pm, n_rows, rb, n_cols, scale, tw);
   PMS_LAUNCH_CHECK();
 }
 
@@ -564,7 +592,10 @@ void fft_psf_spectrum(const float* psf, int q, int p, float2* spec, float2* work
   const int pm = ilog2(p);
   const int L = p / 2;
   // work holds the (P/2+1) x P row spectra, column-major
-  launch_rows_r2c(psf, q, q / 2, pm, p, work, p, tw, nullptr, st);
+  R2cBatch rb{};
+  rb.src[0] = psf;
+  rb.T[0] = work;
+  launch_rows_r2c(rb, 1, q, q / 2, pm, p, p, tw, st);
   const int cpb = cols_per_block(p);
   const size_t lds2 = (size_t)cpb * (p + (p >> 5)) * sizeof(float2);
   PMS_FFT_DISPATCH(first_radix(pm), k_cols_fwd, dim3(ceil_div(L + 1, cpb)), dim3(kThreads), lds2, st,
@@ -574,21 +605,48 @@ void fft_psf_spectrum(const float* psf, int q, int p, float2* spec, float2* work
 
 void fft_conv(const float* in, int n, int p, const float2* spec, float* out, float2* work, const int* pred,
               hipStream_t st) {
+  const FftConvJob j{in, spec, out, work, pred};
+  fft_conv_batch(&j, 1, n, p, st);
+}
+
+// Jobs are independent: job j's blocks read and write only job j's buffers, so a batch gives each
+// job exactly the result of its own single-job call.
+void fft_conv_batch(const FftConvJob* jobs, int count, int n, int p, hipStream_t st) {
   check_fft_len(p);
   PMS_REQUIRE(p == n || p == 2 * n, "fft_conv: frame must be N (circular) or 2N (linear)");
   const float2* tw = fft_twiddles();
   const int pm = ilog2(p);
   const int L = p / 2;
-  float2* T1 = work;
-  float2* T2 = work + (size_t)(L + 1) * n;
-  launch_rows_r2c(in, n, 0, pm, n, T1, n, tw, pred, st);
   const int cpb = cols_per_block(p);
   const size_t lds2 = (size_t)cpb * (p + (p >> 5)) * sizeof(float2);
-  PMS_FFT_DISPATCH(first_radix(pm), k_cols_conv, dim3(ceil_div(L + 1, cpb)), dim3(kThreads), lds2, st,
-                   (const float2*)T1, n, pm, spec, T2, n, cpb, tw, pred);
-  PMS_LAUNCH_CHECK();
   const float scale = 1.0f / ((float)p * (float)p);
-  launch_rows_c2r((const float2*)T2, n, pm, n, out, n, scale, tw, pred, st);
+  for (int j0 = 0; j0 < count; j0 += kMaxBatch) {
+    const int nb = std::min(kMaxBatch, count - j0);
+    R2cBatch rb{};
+    ColBatch cb{};
+    C2rBatch ob{};
+    for (int k = 0; k < nb; ++k) {
+      const FftConvJob& j = jobs[j0 + k];
+      PMS_REQUIRE(j.in && j.spec && j.out && j.work, "fft_conv: NULL buffer");
+      float2* T1 = j.work;
+      float2* T2 = j.work + (size_t)(L + 1) * n;
+      rb.src[k] = j.in;
+      rb.T[k] = T1;
+      rb.pred[k] = j.pred;
+      cb.T1[k] = T1;
+      cb.S[k] = j.spec;
+      cb.T2[k] = T2;
+      cb.pred[k] = j.pred;
+      ob.T2[k] = T2;
+      ob.out[k] = j.out;
+      ob.pred[k] = j.pred;
+    }
+    launch_rows_r2c(rb, nb, n, 0, pm, n, n, tw, st);
+    PMS_FFT_DISPATCH(first_radix(pm), k_cols_conv, dim3(ceil_div(L + 1, cpb), nb), dim3(kThreads), lds2, st, cb, n,
+                     pm, n, cpb, tw);
+    PMS_LAUNCH_CHECK();
+    launch_rows_c2r(ob, nb, n, pm, n, n, scale, tw, st);
+  }
 }
 
 // numpy rfft2 spectrum (rows = axis-0 frequency m in [0,P), cols = k in [0,P/2]) of an

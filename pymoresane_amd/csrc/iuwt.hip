@@ -24,6 +24,9 @@
 // with period 2N, the two passes compose into one circular filter).  The
 // detail epilogue optionally multiplies by a {0,1} byte mask and/or reduces
 // sum((E - w)^2) against a reference plane instead of storing w.
+#include <algorithm>
+#include <vector>
+
 #include "pms_internal.h"
 
 namespace pms {
@@ -35,7 +38,7 @@ __constant__ float c_h5[5] = {1.f / 16, 4.f / 16, 6.f / 16, 4.f / 16, 1.f / 16};
 __constant__ float c_h9[9] = {1.f / 256, 8.f / 256, 28.f / 256, 56.f / 256, 70.f / 256,
                               56.f / 256, 28.f / 256, 8.f / 256, 1.f / 256};
 
-struct ScaleParams {
+struct ScaleJob {
   const float* in;        // C0 (decomp/smooth) or R (recomp)
   float* out_c;           // C (decomp/smooth) or R' (recomp); may be null in decomp
   float* out_w;           // decomp: detail output (null = not stored)
@@ -44,6 +47,12 @@ struct ScaleParams {
   const float* ref;       // decomp: reduce sum((ref - w)^2) into partial[] (null = off)
   double* partial;        // one double per workgroup when ref != null
   const int* pred;        // skip the launch body when *pred == 0 (null = always run)
+};
+
+// One launch serves up to kMaxBatch jobs of the same shape and scale (grid y, or z for the
+// small-plane kernel, selects the job).
+struct ScaleParams {
+  ScaleJob job[kMaxBatch];
   int h, w;               // plane shape (row-major, contiguous)
   int d;                  // dilation 2^s
   int q_log2, tj;         // comb: Q = 1<<q_log2 phases x TJ teeth (Q*TJ <= 256)
@@ -67,15 +76,16 @@ struct ModeTraits<IUWT_RECOMP> {
 };
 
 template <int MODE, int PF, int ML>
-__global__ __launch_bounds__(kThreads) void k_iuwt_scale(ScaleParams a) {
+__global__ __launch_bounds__(kThreads) void k_iuwt_scale(ScaleParams sp) {
+  const ScaleJob a = sp.job[blockIdx.y];
   constexpr int kMaxLoads = ML;  // LDS-staging loads per thread per comb row
   constexpr int H = ModeTraits<MODE>::H;
   constexpr int WA = (MODE == IUWT_DECOMP) ? 7 : 5;  // window of horizontal 5-tap results
   if (a.pred && *a.pred == 0) return;
 
   extern __shared__ float lds[];
-  const int Q = 1 << a.q_log2;
-  const int TJ = a.tj;
+  const int Q = 1 << sp.q_log2;
+  const int TJ = sp.tj;
   const int row_elems = (TJ + 2 * H) * Q;
   float* buf0 = lds;
   float* buf1 = lds + row_elems;
@@ -83,28 +93,28 @@ __global__ __launch_bounds__(kThreads) void k_iuwt_scale(ScaleParams a) {
   // decode the (remapped) block id: consecutive logical ids walk consecutive segments of one strip
   const unsigned nblk = gridDim.x;
   unsigned b = xcd_remap(blockIdx.x, nblk);
-  const int seg = b % a.n_seg;
-  b /= a.n_seg;
-  const int tb = b % a.n_tb;
-  b /= a.n_tb;
-  const int cg = b % a.n_cg;
-  const int p = b / a.n_cg;  // row phase
-  const int d = a.d;
+  const int seg = b % sp.n_seg;
+  b /= sp.n_seg;
+  const int tb = b % sp.n_tb;
+  b /= sp.n_tb;
+  const int cg = b % sp.n_cg;
+  const int p = b / sp.n_cg;  // row phase
+  const int d = sp.d;
 
   const int t = threadIdx.x;
   const int q = t & (Q - 1);
-  const int j = t >> a.q_log2;
+  const int j = t >> sp.q_log2;
   const bool thread_live = j < TJ;
   const int col = cg * Q + q + (tb * TJ + j) * d;
-  const bool col_ok = thread_live && col < a.w;
+  const bool col_ok = thread_live && col < sp.w;
 
   // rows of this phase: r = p + i*d for i in [0, mr)
-  const int mr = (a.h - p + d - 1) / d;
-  const int o_begin = seg * a.k_rows;
-  const int o_end = min(o_begin + a.k_rows, mr);
+  const int mr = (sp.h - p + d - 1) / d;
+  const int o_begin = seg * sp.k_rows;
+  const int o_end = min(o_begin + sp.k_rows, mr);
   const int i_begin = o_begin - H;
   const int n_steps = (o_end - o_begin) + 2 * H;
-  const int w = a.w;
+  const int w = sp.w;
 
   // per-thread staging slots for one comb row
   int ld_col[kMaxLoads];
@@ -113,7 +123,7 @@ __global__ __launch_bounds__(kThreads) void k_iuwt_scale(ScaleParams a) {
     const int e = t + m * kThreads;
     ld_col[m] = -1;
     if (e < row_elems) {
-      const int jj = e >> a.q_log2;
+      const int jj = e >> sp.q_log2;
       const int qq = e & (Q - 1);
       ld_col[m] = reflect_idx(cg * Q + qq + (tb * TJ + jj - H) * d, w);
     }
@@ -121,7 +131,7 @@ __global__ __launch_bounds__(kThreads) void k_iuwt_scale(ScaleParams a) {
 
   float regs[PF][kMaxLoads];
   auto load_row = [&](float (&dst)[kMaxLoads], int i) {
-    const int r = reflect_idx(p + i * d, a.h);
+    const int r = reflect_idx(p + i * d, sp.h);
     const float* src = a.in + (size_t)r * w;
 #pragma unroll
     for (int m = 0; m < kMaxLoads; ++m)
@@ -228,11 +238,12 @@ __global__ __launch_bounds__(kThreads) void k_iuwt_scale(ScaleParams a) {
 // 9x9 (decomposition) or 5x5 neighbourhood at once, so a scale costs one memory latency instead
 // of a row walk.  Same arithmetic as k_iuwt_scale: row sums first, then the vertical combination.
 template <int MODE>
-__global__ __launch_bounds__(kThreads) void k_iuwt_small(ScaleParams a) {
+__global__ __launch_bounds__(kThreads) void k_iuwt_small(ScaleParams sp) {
+  const ScaleJob a = sp.job[blockIdx.z];
   constexpr int H = ModeTraits<MODE>::H;
   constexpr int T = 2 * H + 1;
   if (a.pred && *a.pred == 0) return;
-  const int w = a.w, h = a.h, d = a.d;
+  const int w = sp.w, h = sp.h, d = sp.d;
   const int col0 = blockIdx.x * 64 + (threadIdx.x & 63);
   const int row0 = blockIdx.y * 4 + (threadIdx.x >> 6);
   const bool live = col0 < w && row0 < h;
@@ -354,9 +365,9 @@ Plan make_plan(int h, int w, int scale, int halo) {
 }
 
 template <int MODE, int ML>
-void launch_pf(ScaleParams& sp, const Plan& pl, hipStream_t st) {
+void launch_pf(ScaleParams& sp, int count, const Plan& pl, hipStream_t st) {
   const int pf = tuning().pf;
-  dim3 grid(pl.blocks), block(kThreads);
+  dim3 grid(pl.blocks, count), block(kThreads);
   if (pf <= 1)
     hipLaunchKernelGGL((k_iuwt_scale<MODE, 1, ML>), grid, block, pl.lds_bytes, st, sp);
   else if (pf == 2)
@@ -371,14 +382,14 @@ void launch_pf(ScaleParams& sp, const Plan& pl, hipStream_t st) {
 }
 
 template <int MODE>
-void launch_mode(ScaleParams& sp, const Plan& pl, int halo, hipStream_t st) {
+void launch_mode(ScaleParams& sp, int count, const Plan& pl, int halo, hipStream_t st) {
   const int elems = (pl.tj + 2 * halo) << pl.q_log2;
   if (elems <= 2 * kThreads)
-    launch_pf<MODE, 2>(sp, pl, st);
+    launch_pf<MODE, 2>(sp, count, pl, st);
   else if (elems <= 4 * kThreads)
-    launch_pf<MODE, 4>(sp, pl, st);
+    launch_pf<MODE, 4>(sp, count, pl, st);
   else
-    launch_pf<MODE, 8>(sp, pl, st);
+    launch_pf<MODE, 8>(sp, count, pl, st);
 }
 
 }  // namespace
@@ -395,59 +406,72 @@ bool use_small(int h, int w) { return (long)h * w <= tuning().small_pixels; }
 
 dim3 small_grid(int h, int w) { return dim3(ceil_div(w, 64), ceil_div(h, 4)); }
 
-int iuwt_scale(const IuwtScaleCall& c, hipStream_t st) {
-  check_iuwt_scale(c.h, c.w, c.scale);
-  if (use_small(c.h, c.w)) {
-    ScaleParams sp{};
-    sp.in = c.in;
-    sp.out_c = c.out_c;
-    sp.out_w = c.out_w;
-    sp.w_in = c.w_in;
-    sp.mask = c.mask;
-    sp.ref = c.ref;
-    sp.partial = c.partial;
-    sp.pred = c.pred;
-    sp.h = c.h;
-    sp.w = c.w;
-    sp.d = 1 << c.scale;
-    const dim3 grid = small_grid(c.h, c.w);
-    if (c.mode == IUWT_DECOMP)
-      hipLaunchKernelGGL(k_iuwt_small<IUWT_DECOMP>, grid, dim3(kThreads), 0, st, sp);
-    else if (c.mode == IUWT_SMOOTH)
-      hipLaunchKernelGGL(k_iuwt_small<IUWT_SMOOTH>, grid, dim3(kThreads), 0, st, sp);
-    else
-      hipLaunchKernelGGL(k_iuwt_small<IUWT_RECOMP>, grid, dim3(kThreads), 0, st, sp);
-    PMS_LAUNCH_CHECK();
-    return (int)(grid.x * grid.y);
+static ScaleJob scale_job(const IuwtScaleCall& c) {
+  ScaleJob j;
+  j.in = c.in;
+  j.out_c = c.out_c;
+  j.out_w = c.out_w;
+  j.w_in = c.w_in;
+  j.mask = c.mask;
+  j.ref = c.ref;
+  j.partial = c.partial;
+  j.pred = c.pred;
+  return j;
+}
+
+int iuwt_scale(const IuwtScaleCall& c, hipStream_t st) { return iuwt_scale_batch(&c, 1, st); }
+
+// All calls share mode, shape and scale; returns the blocks (partials) per job.
+int iuwt_scale_batch(const IuwtScaleCall* calls, int count, hipStream_t st) {
+  PMS_REQUIRE(count >= 1, "empty IUWT batch");
+  const IuwtScaleCall& c0 = calls[0];
+  check_iuwt_scale(c0.h, c0.w, c0.scale);
+  for (int k = 1; k < count; ++k)
+    PMS_REQUIRE(calls[k].mode == c0.mode && calls[k].h == c0.h && calls[k].w == c0.w && calls[k].scale == c0.scale,
+                "IUWT batch mixes modes, shapes or scales");
+  const bool small = use_small(c0.h, c0.w);
+  const int halo = (c0.mode == IUWT_DECOMP) ? 4 : 2;
+  Plan pl{};
+  if (!small) {
+    pl = make_plan(c0.h, c0.w, c0.scale, halo);
+    PMS_REQUIRE((pl.tj + 2 * halo) * (1 << pl.q_log2) <= 8 * kThreads, "IUWT tile too wide");
   }
-  const int halo = (c.mode == IUWT_DECOMP) ? 4 : 2;
-  Plan pl = make_plan(c.h, c.w, c.scale, halo);
-  PMS_REQUIRE((pl.tj + 2 * halo) * (1 << pl.q_log2) <= 8 * kThreads, "IUWT tile too wide");
-  ScaleParams sp;
-  sp.in = c.in;
-  sp.out_c = c.out_c;
-  sp.out_w = c.out_w;
-  sp.w_in = c.w_in;
-  sp.mask = c.mask;
-  sp.ref = c.ref;
-  sp.partial = c.partial;
-  sp.pred = c.pred;
-  sp.h = c.h;
-  sp.w = c.w;
-  sp.d = 1 << c.scale;
+  ScaleParams sp{};
+  sp.h = c0.h;
+  sp.w = c0.w;
+  sp.d = 1 << c0.scale;
   sp.q_log2 = pl.q_log2;
   sp.tj = pl.tj;
   sp.k_rows = pl.k_rows;
   sp.n_cg = pl.n_cg;
   sp.n_tb = pl.n_tb;
   sp.n_seg = pl.n_seg;
-  if (c.mode == IUWT_DECOMP)
-    launch_mode<IUWT_DECOMP>(sp, pl, halo, st);
-  else if (c.mode == IUWT_SMOOTH)
-    launch_mode<IUWT_SMOOTH>(sp, pl, halo, st);
-  else
-    launch_mode<IUWT_RECOMP>(sp, pl, halo, st);
-  return pl.blocks;
+  int blocks = 0;
+  for (int j0 = 0; j0 < count; j0 += kMaxBatch) {
+    const int nb = std::min(kMaxBatch, count - j0);
+    for (int k = 0; k < nb; ++k) sp.job[k] = scale_job(calls[j0 + k]);
+    if (small) {
+      const dim3 g2 = small_grid(c0.h, c0.w);
+      const dim3 grid(g2.x, g2.y, nb);
+      if (c0.mode == IUWT_DECOMP)
+        hipLaunchKernelGGL(k_iuwt_small<IUWT_DECOMP>, grid, dim3(kThreads), 0, st, sp);
+      else if (c0.mode == IUWT_SMOOTH)
+        hipLaunchKernelGGL(k_iuwt_small<IUWT_SMOOTH>, grid, dim3(kThreads), 0, st, sp);
+      else
+        hipLaunchKernelGGL(k_iuwt_small<IUWT_RECOMP>, grid, dim3(kThreads), 0, st, sp);
+      PMS_LAUNCH_CHECK();
+      blocks = (int)(g2.x * g2.y);
+    } else {
+      if (c0.mode == IUWT_DECOMP)
+        launch_mode<IUWT_DECOMP>(sp, nb, pl, halo, st);
+      else if (c0.mode == IUWT_SMOOTH)
+        launch_mode<IUWT_SMOOTH>(sp, nb, pl, halo, st);
+      else
+        launch_mode<IUWT_RECOMP>(sp, nb, pl, halo, st);
+      blocks = pl.blocks;
+    }
+  }
+  return blocks;
 }
 
 int iuwt_scale_blocks(int h, int w, int scale, int mode) {
@@ -465,49 +489,76 @@ int iuwt_scale_blocks(int h, int w, int scale, int mode) {
 long iuwt_decompose(const float* in, int h, int w, int scale_count, int scale_adjust, float* detail,
                     float* smoothed, float* work, const uint8_t* mask, const float* ref,
                     double* partial, const int* pred, hipStream_t st) {
-  PMS_REQUIRE(scale_adjust >= 0 && scale_count > scale_adjust, "need scale_count > scale_adjust >= 0");
-  check_iuwt_scale(h, w, scale_count - 1);
+  IuwtDecompJob j{in, scale_count, scale_adjust, detail, smoothed, work, mask, ref, partial, pred, 0};
+  iuwt_decompose_batch(&j, 1, h, w, st);
+  return j.np;
+}
+
+// Scale by scale over all jobs: at scale s a job either pre-smooths (s < scale_adjust) or
+// decomposes (scale_adjust <= s < scale_count); jobs of the same kind share one launch.
+void iuwt_decompose_batch(IuwtDecompJob* jobs, int count, int h, int w, hipStream_t st) {
   const size_t plane = (size_t)h * w;
-  float* ping[2] = {work, work + plane};
-  const float* cur = in;
-  int slot = 0;
-  for (int s = 0; s < scale_adjust; ++s) {
-    IuwtScaleCall c{};
-    c.mode = IUWT_SMOOTH;
-    c.in = cur;
-    c.out_c = ping[slot];
-    c.h = h;
-    c.w = w;
-    c.scale = s;
-    c.pred = pred;
-    iuwt_scale(c, st);
-    cur = ping[slot];
-    slot ^= 1;
+  struct State {
+    const float* cur;
+    int slot;
+  };
+  std::vector<State> state(count);
+  int smax = 0;
+  for (int k = 0; k < count; ++k) {
+    IuwtDecompJob& j = jobs[k];
+    PMS_REQUIRE(j.scale_adjust >= 0 && j.scale_count > j.scale_adjust, "need scale_count > scale_adjust >= 0");
+    check_iuwt_scale(h, w, j.scale_count - 1);
+    state[k] = {j.in, 0};
+    j.np = 0;
+    smax = std::max(smax, j.scale_count);
   }
-  long np = 0;
-  for (int s = scale_adjust; s < scale_count; ++s) {
-    const int k = s - scale_adjust;
-    const bool last = (s == scale_count - 1);
-    IuwtScaleCall c{};
-    c.mode = IUWT_DECOMP;
-    c.in = cur;
-    c.out_c = last ? smoothed : ping[slot];
-    c.out_w = detail ? detail + (size_t)k * plane : nullptr;
-    c.mask = mask ? mask + (size_t)k * plane : nullptr;
-    c.ref = ref ? ref + (size_t)k * plane : nullptr;
-    c.partial = ref ? partial + np : nullptr;
-    c.h = h;
-    c.w = w;
-    c.scale = s;
-    c.pred = pred;
-    const int nb = iuwt_scale(c, st);
-    if (ref) np += nb;
-    if (!last) {
-      cur = ping[slot];
-      slot ^= 1;
+  std::vector<IuwtScaleCall> calls;
+  std::vector<int> who;
+  calls.reserve(count);
+  who.reserve(count);
+  for (int s = 0; s < smax; ++s) {
+    for (int mode : {IUWT_SMOOTH, IUWT_DECOMP}) {
+      calls.clear();
+      who.clear();
+      for (int k = 0; k < count; ++k) {
+        const IuwtDecompJob& j = jobs[k];
+        float* ping = j.work + (size_t)state[k].slot * plane;
+        IuwtScaleCall c{};
+        c.mode = mode;
+        c.in = state[k].cur;
+        c.h = h;
+        c.w = w;
+        c.scale = s;
+        c.pred = j.pred;
+        if (mode == IUWT_SMOOTH) {
+          if (s >= j.scale_adjust) continue;
+          c.out_c = ping;
+        } else {
+          if (s < j.scale_adjust || s >= j.scale_count) continue;
+          const int kk = s - j.scale_adjust;
+          const bool last = (s == j.scale_count - 1);
+          c.out_c = last ? j.smoothed : ping;
+          c.out_w = j.detail ? j.detail + (size_t)kk * plane : nullptr;
+          c.mask = j.mask ? j.mask + (size_t)kk * plane : nullptr;
+          c.ref = j.ref ? j.ref + (size_t)kk * plane : nullptr;
+          c.partial = j.ref ? j.partial + j.np : nullptr;
+        }
+        calls.push_back(c);
+        who.push_back(k);
+      }
+      if (calls.empty()) continue;
+      const int nb = iuwt_scale_batch(calls.data(), (int)calls.size(), st);
+      for (size_t i = 0; i < who.size(); ++i) {
+        IuwtDecompJob& j = jobs[who[i]];
+        State& sk = state[who[i]];
+        if (mode == IUWT_DECOMP && j.ref) j.np += nb;
+        if (mode == IUWT_SMOOTH || s != j.scale_count - 1) {
+          sk.cur = j.work + (size_t)sk.slot * plane;
+          sk.slot ^= 1;
+        }
+      }
     }
   }
-  return np;
 }
 
 long iuwt_decompose_partials(int h, int w, int scale_count, int scale_adjust) {
@@ -522,60 +573,80 @@ long iuwt_decompose_partials(int h, int w, int scale_count, int scale_adjust) {
 void iuwt_recompose(const float* detail, int nplanes, int h, int w, int scale_adjust,
                     const float* smoothed, float* out, float* work, const uint8_t* mask,
                     const int* pred, hipStream_t st) {
-  PMS_REQUIRE(nplanes >= 1 && scale_adjust >= 0, "recomposition needs at least one detail plane");
-  const int top = nplanes + scale_adjust;
-  check_iuwt_scale(h, w, top - 1);
+  const IuwtRecompJob j{detail, nplanes, scale_adjust, smoothed, out, work, mask, pred};
+  iuwt_recompose_batch(&j, 1, h, w, st);
+}
+
+// From the coarsest scale down: at scale s a job adds its detail plane (scale_adjust <= s) or only
+// smooths (s < scale_adjust); jobs of the same kind share one launch.  Without a smoothed base R
+// starts at zero, so a job's coarsest step is its (masked) detail plane itself.
+void iuwt_recompose_batch(const IuwtRecompJob* jobs, int count, int h, int w, hipStream_t st) {
   const size_t plane = (size_t)h * w;
-  float* ping[2] = {work, work + plane};
-  int slot = 0;
-  const float* cur = smoothed;
-  int s = top - 1;
-  if (!smoothed) {
-    // R starts at zero, so the coarsest step is the detail plane itself
-    if (!mask) {
-      cur = detail + (size_t)(s - scale_adjust) * plane;
-    } else {
-      masked_copy(detail + (size_t)(s - scale_adjust) * plane, mask + (size_t)(s - scale_adjust) * plane,
-                  ping[slot], plane, pred, st);
-      cur = ping[slot];
-      slot ^= 1;
+  struct State {
+    const float* cur;
+    int slot, s_start, remaining;
+  };
+  std::vector<State> state(count);
+  int smax = -1;
+  for (int k = 0; k < count; ++k) {
+    const IuwtRecompJob& j = jobs[k];
+    PMS_REQUIRE(j.nplanes >= 1 && j.scale_adjust >= 0, "recomposition needs at least one detail plane");
+    const int top = j.nplanes + j.scale_adjust;
+    check_iuwt_scale(h, w, top - 1);
+    State sk{j.smoothed, 0, top - 1, 0};
+    if (!j.smoothed) {
+      const size_t off = (size_t)(top - 1 - j.scale_adjust) * plane;
+      if (!j.mask) {
+        sk.cur = j.detail + off;
+      } else {
+        masked_copy(j.detail + off, j.mask + off, j.work, plane, j.pred, st);
+        sk.cur = j.work;
+        sk.slot = 1;
+      }
+      --sk.s_start;
     }
-    --s;
+    sk.remaining = sk.s_start + 1;  // launches still to issue
+    if (sk.remaining == 0) copy_plane(sk.cur, j.out, plane, j.pred, st);
+    state[k] = sk;
+    smax = std::max(smax, sk.s_start);
   }
-  int remaining = (s - scale_adjust + 1) + scale_adjust;  // launches still to issue
-  if (remaining == 0) {
-    copy_plane(cur, out, plane, pred, st);
-    return;
-  }
-  for (; s >= scale_adjust; --s) {
-    --remaining;
-    IuwtScaleCall c{};
-    c.mode = IUWT_RECOMP;
-    c.in = cur;
-    c.out_c = remaining == 0 ? out : ping[slot];
-    c.w_in = detail + (size_t)(s - scale_adjust) * plane;
-    c.mask = mask ? mask + (size_t)(s - scale_adjust) * plane : nullptr;
-    c.h = h;
-    c.w = w;
-    c.scale = s;
-    c.pred = pred;
-    iuwt_scale(c, st);
-    cur = c.out_c;
-    slot ^= 1;
-  }
-  for (s = scale_adjust - 1; s >= 0; --s) {
-    --remaining;
-    IuwtScaleCall c{};
-    c.mode = IUWT_SMOOTH;
-    c.in = cur;
-    c.out_c = remaining == 0 ? out : ping[slot];
-    c.h = h;
-    c.w = w;
-    c.scale = s;
-    c.pred = pred;
-    iuwt_scale(c, st);
-    cur = c.out_c;
-    slot ^= 1;
+  std::vector<IuwtScaleCall> calls;
+  std::vector<int> who;
+  calls.reserve(count);
+  who.reserve(count);
+  for (int s = smax; s >= 0; --s) {
+    for (int mode : {IUWT_RECOMP, IUWT_SMOOTH}) {
+      calls.clear();
+      who.clear();
+      for (int k = 0; k < count; ++k) {
+        const IuwtRecompJob& j = jobs[k];
+        const State& sk = state[k];
+        if (s > sk.s_start) continue;
+        if ((mode == IUWT_RECOMP) != (s >= j.scale_adjust)) continue;
+        IuwtScaleCall c{};
+        c.mode = mode;
+        c.in = sk.cur;
+        c.out_c = sk.remaining == 1 ? j.out : j.work + (size_t)sk.slot * plane;
+        if (mode == IUWT_RECOMP) {
+          c.w_in = j.detail + (size_t)(s - j.scale_adjust) * plane;
+          c.mask = j.mask ? j.mask + (size_t)(s - j.scale_adjust) * plane : nullptr;
+        }
+        c.h = h;
+        c.w = w;
+        c.scale = s;
+        c.pred = j.pred;
+        calls.push_back(c);
+        who.push_back(k);
+      }
+      if (calls.empty()) continue;
+      iuwt_scale_batch(calls.data(), (int)calls.size(), st);
+      for (size_t i = 0; i < who.size(); ++i) {
+        State& sk = state[who[i]];
+        sk.cur = calls[i].out_c;
+        sk.slot ^= 1;
+        --sk.remaining;
+      }
+    }
   }
 }
 

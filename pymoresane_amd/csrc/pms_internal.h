@@ -71,6 +71,12 @@ __device__ __forceinline__ float block_min_f32(float v) {
   return m;
 }
 
+// ------------------------------------------------------------------- batching
+// Kernels that can serve several jobs of one geometry in a single launch take
+// their per-job pointers from fixed-capacity arrays in the kernel arguments;
+// one grid dimension selects the job.  A single-job call is a batch of one.
+constexpr int kMaxBatch = 32;
+
 // ----------------------------------------------------------------------- IUWT
 enum IuwtMode { IUWT_DECOMP = 0, IUWT_SMOOTH = 1, IUWT_RECOMP = 2 };
 
@@ -97,6 +103,33 @@ long iuwt_decompose_partials(int h, int w, int scale_count, int scale_adjust);
 void iuwt_recompose(const float* detail, int nplanes, int h, int w, int scale_adjust,
                     const float* smoothed, float* out, float* work, const uint8_t* mask,
                     const int* pred, hipStream_t st);
+// calls of one mode, shape and scale for several jobs, in launches of up to kMaxBatch jobs
+int iuwt_scale_batch(const IuwtScaleCall* calls, int count, hipStream_t st);
+// iuwt_decompose / iuwt_recompose for several jobs of one plane shape; every job keeps its own
+// scale range, buffers, mask and predicate.  Decomposition jobs report their partial count in np.
+struct IuwtDecompJob {
+  const float* in;
+  int scale_count, scale_adjust;
+  float* detail;
+  float* smoothed;
+  float* work;
+  const uint8_t* mask;
+  const float* ref;
+  double* partial;
+  const int* pred;
+  long np;
+};
+void iuwt_decompose_batch(IuwtDecompJob* jobs, int count, int h, int w, hipStream_t st);
+struct IuwtRecompJob {
+  const float* detail;
+  int nplanes, scale_adjust;
+  const float* smoothed;
+  float* out;
+  float* work;
+  const uint8_t* mask;
+  const int* pred;
+};
+void iuwt_recompose_batch(const IuwtRecompJob* jobs, int count, int h, int w, hipStream_t st);
 
 // ------------------------------------------------------------------------ FFT
 // Device twiddle table exp(-2 pi i k / kFftMaxLen), k in [0, kFftMaxLen).
@@ -115,6 +148,15 @@ void fft_psf_spectrum(const float* psf, int q, int p, float2* spec, float2* work
 // out (n x n) = image (n x n) convolved with the PSF of spectrum spec (p = n or 2n).
 void fft_conv(const float* in, int n, int p, const float2* spec, float* out, float2* work,
               const int* pred, hipStream_t st);
+// fft_conv for several jobs of one image side and frame (one launch per pass per kMaxBatch jobs)
+struct FftConvJob {
+  const float* in;
+  const float2* spec;
+  float* out;
+  float2* work;
+  const int* pred;
+};
+void fft_conv_batch(const FftConvJob* jobs, int count, int n, int p, hipStream_t st);
 // spectrum -> digit-reversed column-major layout (host arrays, numpy rfft2 convention)
 void fft_spectrum_from_numpy_host(const float* re_im, int p, float2* spec_host);
 int fft_perm(int k, int p);  // storage position of frequency k in a length-p column
@@ -156,6 +198,13 @@ void source_extract(const float* w, int S, int h, int wd, double tol, int neg_co
                     int* labels_out, void* work, const int* pred, hipStream_t st);
 void dot_partials(const float* a, const float* b, size_t n, double* partial, const int* pred, hipStream_t st);
 void sum_partials(const double* partial, long np, int groups, double* out, const int* pred, hipStream_t st);
+// one group of partials per job: out = sum(partial[0..np))
+struct SumJob {
+  const double* partial;
+  long np;
+  double* out;
+};
+void sum_partials_batch(const SumJob* jobs, int count, hipStream_t st);
 void plane_sumsq_partials(const float* a, int S, size_t plane, double* partial, const int* pred, hipStream_t st);
 void mean_std(const float* a, size_t n, double* partial, double* sum_out, double* dev2_out, hipStream_t st);
 

@@ -1,18 +1,25 @@
 // Continuous-batching deconvolution server.
 //
-// Requests (host-resident dirty/PSF, host outputs) enter a FIFO queue.  A pool
-// of worker threads, each owning a HIP stream and a grow-only device job
-// workspace, admits the request at the head of the queue as soon as a worker
-// is free and the sum of the device footprints of running requests plus the
-// new one fits the HBM budget.  Admitted requests run concurrently on their
-// own streams, so work that arrives while others are on the GPU overlaps with
-// them.  Every request's result is the same as running it alone: the kernels
-// are deterministic and share no state across streams.
+// Requests (host-resident dirty/PSF, host outputs) join the FIFO queue of their
+// size class (image side, PSF side, subregion, convolution mode).  Each class
+// has a host thread with its own HIP stream and a Runner of max_batch slots
+// that steps its admitted requests in lockstep.  Between steps the thread
+// admits queued requests while a slot is free and the device footprints of all
+// running requests (every class) plus the new one fit the HBM budget, so a
+// request that arrives while others run joins the running batch at the next
+// step, and classes overlap on the device through their streams.  Every
+// request's result is the same as running it alone: each job's arithmetic is
+// independent of the others in its batch and no state is shared across
+// streams.
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <map>
+#include <memory>
 #include <mutex>
 #include <thread>
+#include <tuple>
 
 #include "engine.h"
 
@@ -20,9 +27,9 @@ namespace pms {
 
 class Server {
  public:
-  Server(int n_workers, size_t budget) : budget_(budget), t0_(std::chrono::steady_clock::now()) {
+  Server(int max_batch, size_t budget)
+      : max_batch_(max_batch), budget_(budget), t0_(std::chrono::steady_clock::now()) {
     PMS_HIP(hipGetDevice(&dev_));
-    for (int i = 0; i < n_workers; ++i) workers_.emplace_back([this] { worker(); });
   }
 
   ~Server() {
@@ -31,7 +38,7 @@ class Server {
       stop_ = true;
     }
     cv_.notify_all();
-    for (auto& t : workers_) t.join();
+    for (auto& kv : lanes_) kv.second->th.join();
   }
 
   double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count(); }
@@ -42,11 +49,20 @@ class Server {
                       (r->mask_host ? (size_t)r->prm.n * r->prm.n * sizeof(float) : 0);
     r->footprint = job_footprint_bytes(g) + io;
     PMS_REQUIRE(r->footprint <= budget_, "request footprint exceeds the server's HBM budget");
+    // max_batch 1 is the plain path: one lane, one request at a time
+    const Key key = max_batch_ == 1 ? Key{0, 0, 0, 0} : std::make_tuple(g.n, g.psf_n, g.sub, r->prm.conv_mode);
     r->status = 1;  // queued
     r->t_submit = now();
     {
       std::lock_guard<std::mutex> lk(mu_);
-      queue_.push_back(r);
+      auto it = lanes_.find(key);
+      if (it == lanes_.end()) {
+        it = lanes_.emplace(key, std::make_unique<Lane>()).first;
+        Lane* lane = it->second.get();
+        lane->free_slots = max_batch_;  // its runner starts with every slot free
+        lane->th = std::thread([this, lane] { lane_loop(*lane); });
+      }
+      it->second->queue.emplace_back(next_seq_++, r);
       ++pending_;
     }
     cv_.notify_all();
@@ -62,12 +78,10 @@ class Server {
     return done_cv_.wait_for(lk, std::chrono::duration<double>(timeout_s), [r] { return r->status <= 0; });
   }
 
-  size_t used() {
-    std::lock_guard<std::mutex> lk(mu_);
-    return used_;
-  }
-
  private:
+  using Key = std::tuple<int, int, int, int>;
+
+  // device copies of one request's inputs and outputs (per slot, grow-only)
   struct IoBuffers {
     DeviceArena arena;
     size_t n2 = 0, p2 = 0;
@@ -87,63 +101,111 @@ class Server {
     }
   };
 
-  void worker() {
+  struct Lane {
+    std::thread th;
+    std::deque<std::pair<long, pms_serve_request*>> queue;  // (submission number, request)
+    int free_slots = 0;
+  };
+
+  // The head of a lane is admitted when its lane has a free slot and its footprint fits the budget.
+  // Budget is granted in submission order: an older request that has a slot but waits for budget
+  // in another lane holds back younger ones, so large requests are not starved by small ones.
+  bool admissible(const Lane& lane, int free_slots) const {
+    if (free_slots <= 0 || lane.queue.empty()) return false;
+    const long seq = lane.queue.front().first;
+    for (auto& kv : lanes_) {
+      const Lane& o = *kv.second;
+      if (&o == &lane || o.queue.empty() || o.free_slots <= 0) continue;
+      if (o.queue.front().first < seq) return false;
+    }
+    return used_ == 0 || used_ + lane.queue.front().second->footprint <= budget_;
+  }
+
+  void lane_loop(Lane& lane) {
     if (hipSetDevice(dev_) != hipSuccess) return;
-    hipStream_t st = nullptr;
-    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return;
-    {
-      Job job(st);
-      IoBuffers io;
-      for (;;) {
-        pms_serve_request* r = nullptr;
-        {
-          std::unique_lock<std::mutex> lk(mu_);
-          cv_.wait(lk, [this] {
-            return stop_ || (!queue_.empty() && (used_ == 0 || used_ + queue_.front()->footprint <= budget_));
-          });
-          if (stop_ && queue_.empty()) break;
-          if (queue_.empty()) continue;
-          r = queue_.front();
-          queue_.pop_front();
+    std::unique_ptr<Runner> rn;
+    try {
+      rn = std::make_unique<Runner>(max_batch_);
+    } catch (...) {
+      return;
+    }
+    const hipStream_t st = rn->stream();
+    std::vector<IoBuffers> io(max_batch_);
+    std::vector<pms_serve_request*> slot_req(max_batch_, nullptr);
+    for (;;) {
+      std::vector<pms_serve_request*> admitted;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        lane.free_slots = rn->free_slots();
+        cv_.wait(lk, [&] { return stop_ || !rn->idle() || admissible(lane, lane.free_slots); });
+        if (stop_ && rn->idle() && lane.queue.empty()) break;
+        while (admissible(lane, lane.free_slots)) {
+          pms_serve_request* r = lane.queue.front().second;
+          lane.queue.pop_front();
           used_ += r->footprint;
           r->status = 2;  // running
+          admitted.push_back(r);
+          --lane.free_slots;
         }
+      }
+      if (!admitted.empty()) cv_.notify_all();  // younger requests held back in other lanes re-check
+      for (pms_serve_request* r : admitted) {
         r->t_start = now();
+        const int k = rn->first_free();
         const int n = r->prm.n, pn = r->prm.psf_n;
         const size_t n2 = (size_t)n * n, p2 = (size_t)pn * pn;
-        Trace tr;
-        int complete = 0;
         const int status = run_guarded([&] {
-          io.ensure(n2, p2, r->mask_host != nullptr);
-          PMS_HIP(hipMemcpyAsync(io.dirty, r->dirty_host, n2 * sizeof(float), hipMemcpyHostToDevice, st));
-          PMS_HIP(hipMemcpyAsync(io.psf, r->psf_host, p2 * sizeof(float), hipMemcpyHostToDevice, st));
+          IoBuffers& b = io[k];
+          b.ensure(n2, p2, r->mask_host != nullptr);
+          PMS_HIP(hipMemcpyAsync(b.dirty, r->dirty_host, n2 * sizeof(float), hipMemcpyHostToDevice, st));
+          PMS_HIP(hipMemcpyAsync(b.psf, r->psf_host, p2 * sizeof(float), hipMemcpyHostToDevice, st));
           if (r->mask_host)
-            PMS_HIP(hipMemcpyAsync(io.mask_d, r->mask_host, n2 * sizeof(float), hipMemcpyHostToDevice, st));
-          PMS_HIP(hipMemsetAsync(io.model, 0, n2 * sizeof(float), st));
-          PMS_HIP(hipMemcpyAsync(io.resid, io.dirty, n2 * sizeof(float), hipMemcpyDeviceToDevice, st));
-          if (r->by_scale) {
-            job.moresane_by_scale(r->prm, r->start_scale, r->stop_scale, io.dirty, io.psf,
-                                  r->mask_host ? io.mask_d : nullptr, io.model, io.resid, tr);
-          } else {
-            r->major_iters = job.moresane(r->prm, r->prm.scale_count, io.dirty, io.psf,
-                                          r->mask_host ? io.mask_d : nullptr, io.model, io.resid, &complete, tr);
-          }
-          PMS_HIP(hipMemcpyAsync(r->model_host, io.model, n2 * sizeof(float), hipMemcpyDeviceToHost, st));
-          PMS_HIP(hipMemcpyAsync(r->residual_host, io.resid, n2 * sizeof(float), hipMemcpyDeviceToHost, st));
-          PMS_HIP(hipStreamSynchronize(st));
+            PMS_HIP(hipMemcpyAsync(b.mask_d, r->mask_host, n2 * sizeof(float), hipMemcpyHostToDevice, st));
+          fill_f32(b.model, n2, 0.f, st);
+          PMS_HIP(hipMemcpyAsync(b.resid, b.dirty, n2 * sizeof(float), hipMemcpyDeviceToDevice, st));
+          const TaskSpec spec{r->prm,  b.dirty, b.psf, r->mask_host ? b.mask_d : nullptr, b.model, b.resid,
+                              r->by_scale, r->start_scale, r->stop_scale};
+          rn->add(spec, k);
+          slot_req[k] = r;
         });
-        r->t_done = now();
-        {
-          std::lock_guard<std::mutex> lk(mu_);
-          used_ -= r->footprint;
-          r->status = status;  // 0 = done, < 0 = error
-          --pending_;
+        if (status != 0) finish(r, status);
+      }
+      if (rn->idle()) continue;
+      int step_status = run_guarded([&] { rn->step(); });
+      for (int k = 0; k < max_batch_; ++k) {
+        Task* t = rn->task(k);
+        if (!t) continue;
+        if (step_status == 0 && !t->done()) continue;
+        pms_serve_request* r = slot_req[k];
+        int status = step_status != 0 ? step_status : t->status;
+        if (status == 0) {
+          status = run_guarded([&] {
+            const size_t n2 = (size_t)r->prm.n * r->prm.n;
+            PMS_HIP(hipMemcpyAsync(r->model_host, io[k].model, n2 * sizeof(float), hipMemcpyDeviceToHost, st));
+            PMS_HIP(hipMemcpyAsync(r->residual_host, io[k].resid, n2 * sizeof(float), hipMemcpyDeviceToHost, st));
+            PMS_HIP(hipStreamSynchronize(st));
+          });
+        } else if (step_status == 0) {
+          set_error(t->error);
         }
-        cv_.notify_all();
-        done_cv_.notify_all();
+        r->major_iters = t->major_iters;
+        rn->release(k);
+        slot_req[k] = nullptr;
+        finish(r, status);
       }
     }
-    (void)hipStreamDestroy(st);
+  }
+
+  void finish(pms_serve_request* r, int status) {
+    r->t_done = now();
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      used_ -= r->footprint;
+      r->status = status;  // 0 = done, < 0 = error
+      --pending_;
+    }
+    cv_.notify_all();
+    done_cv_.notify_all();
   }
 
   template <class F>
@@ -161,15 +223,16 @@ class Server {
   }
 
   int dev_ = 0;
+  int max_batch_;
   size_t budget_;
   size_t used_ = 0;
   int pending_ = 0;
+  long next_seq_ = 0;
   bool stop_ = false;
   std::chrono::steady_clock::time_point t0_;
   std::mutex mu_;
   std::condition_variable cv_, done_cv_;
-  std::deque<pms_serve_request*> queue_;
-  std::vector<std::thread> workers_;
+  std::map<Key, std::unique_ptr<Lane>> lanes_;
 };
 
 }  // namespace pms
@@ -192,12 +255,12 @@ static int serve_guarded(F&& f) {
 
 extern "C" {
 
-int pms_server_create(int n_workers, double budget_bytes, void** server) {
+int pms_server_create(int max_batch, double budget_bytes, void** server) {
   return serve_guarded([&] {
     PMS_REQUIRE(server != nullptr, "server must not be NULL");
-    PMS_REQUIRE(n_workers >= 1 && n_workers <= 64, "n_workers must be in 1..64");
+    PMS_REQUIRE(max_batch >= 1 && max_batch <= 1024, "max_batch must be in 1..1024");
     PMS_REQUIRE(budget_bytes > 0, "budget must be positive");
-    *server = new Server(n_workers, (size_t)budget_bytes);
+    *server = new Server(max_batch, (size_t)budget_bytes);
   });
 }
 

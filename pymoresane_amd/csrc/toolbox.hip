@@ -485,6 +485,22 @@ __global__ __launch_bounds__(kT) void k_sum_partials(const double* __restrict__ 
   if (threadIdx.x == 0) out[blockIdx.x] = acc;
 }
 
+struct SumBatch {
+  const double* partial[kMaxBatch];
+  long np[kMaxBatch];
+  double* out[kMaxBatch];
+};
+
+// job blockIdx.x: same summation order as k_sum_partials for one group
+__global__ __launch_bounds__(kT) void k_sum_partials_batch(SumBatch b) {
+  const double* p = b.partial[blockIdx.x];
+  const long np = b.np[blockIdx.x];
+  double acc = 0.0;
+  for (long i = threadIdx.x; i < np; i += kT) acc += p[i];
+  acc = block_sum_f64(acc);
+  if (threadIdx.x == 0) *b.out[blockIdx.x] = acc;
+}
+
 }  // namespace
 
 // ============================================================================= host wrappers
@@ -712,6 +728,20 @@ void dot_partials(const float* a, const float* b, size_t n, double* partial, con
 void sum_partials(const double* partial, long np, int groups, double* out, const int* pred, hipStream_t st) {
   hipLaunchKernelGGL(k_sum_partials, dim3(groups), dim3(kT), 0, st, partial, np, out, pred);
   PMS_LAUNCH_CHECK();
+}
+
+void sum_partials_batch(const SumJob* jobs, int count, hipStream_t st) {
+  for (int j0 = 0; j0 < count; j0 += kMaxBatch) {
+    const int nb = std::min(kMaxBatch, count - j0);
+    SumBatch b{};
+    for (int k = 0; k < nb; ++k) {
+      b.partial[k] = jobs[j0 + k].partial;
+      b.np[k] = jobs[j0 + k].np;
+      b.out[k] = jobs[j0 + k].out;
+    }
+    hipLaunchKernelGGL(k_sum_partials_batch, dim3(nb), dim3(kT), 0, st, b);
+    PMS_LAUNCH_CHECK();
+  }
 }
 
 void plane_sumsq_partials(const float* a, int S, size_t plane, double* partial, const int* pred, hipStream_t st) {

@@ -19,45 +19,72 @@ def say(msg):
     sys.stdout.flush()
 
 
-def pairs():
+SIDES = [64, 128, 64, 256, 128, 64, 1024, 1024]
+
+
+def pairs(sides=SIDES):
     out = []
-    for i, n in enumerate([64, 128, 64, 256, 128, 64]):
+    for i, n in enumerate(sides):
         out.append(synth.make_pair(n, 500 + i, n_points=6 + i, n_blobs=1))
     return out
 
 
-def alone(pairs_):
+def alone(pairs_, by_scale=False):
     res = []
     for d, p in pairs_:
         img = DataImage(d, p, None, 1.0, 1.0)
-        img.moresane(**KW)
+        if by_scale:
+            img.moresane_by_scale(start_scale=1, stop_scale=3, **BYSCALE_KW)
+        else:
+            img.moresane(**KW)
         res.append((img.model, img.residual))
     return res
 
 
+BYSCALE_KW = dict(loop_gain=0.2, major_loop_miter=6, enforce_positivity=True)
+
+
 def test_batch_equals_alone(hip):
-    say("batch vs alone, 6 ragged jobs, 4 workers")
+    say("batch vs alone, 8 ragged jobs in 4 size classes, lockstep batches")
     ps = pairs()
     ref = alone(ps)
     jobs = batch.DeviceJobs(ps, [KW] * len(ps))
-    stats = batch.run_batch(jobs, n_workers=4)
+    stats = batch.run_batch(jobs, n_workers=4, max_batch=32)
     assert all(s[0] == 0 for s in stats)
     for (m, r), (m0, r0) in zip(jobs.results(), ref):
         np.testing.assert_array_equal(m, m0)
         np.testing.assert_array_equal(r, r0)
-    # a second run on the same (reset) device state gives the same bits again
+    # fewer slots than jobs per class (slots refill as jobs finish), one worker for every class
     jobs.reset()
-    batch.run_batch(jobs, n_workers=3, order=[5, 4, 3, 2, 1, 0])
+    batch.run_batch(jobs, n_workers=1, max_batch=2, order=[7, 5, 4, 3, 2, 1, 0, 6])
+    for (m, r), (m0, r0) in zip(jobs.results(), ref):
+        np.testing.assert_array_equal(m, m0)
+        np.testing.assert_array_equal(r, r0)
+    # plain path through the same entry point
+    jobs.reset()
+    batch.run_batch(jobs, n_workers=1, max_batch=1)
+    for (m, r), (m0, r0) in zip(jobs.results(), ref):
+        np.testing.assert_array_equal(m, m0)
+        np.testing.assert_array_equal(r, r0)
+
+
+def test_batch_by_scale_equals_alone(hip):
+    say("batch vs alone, by-scale jobs (scale counts differ across the batch)")
+    ps = pairs([64, 64, 128, 64])
+    ref = alone(ps, by_scale=True)
+    jobs = batch.DeviceJobs(ps, [BYSCALE_KW] * len(ps))
+    stats = batch.run_batch(jobs, n_workers=2, max_batch=8, by_scale=True, start_scale=1, stop_scale=3)
+    assert all(s[0] == 0 for s in stats)
     for (m, r), (m0, r0) in zip(jobs.results(), ref):
         np.testing.assert_array_equal(m, m0)
         np.testing.assert_array_equal(r, r0)
 
 
 def test_served_mix_equals_alone(hip):
-    say("server: staggered arrivals, 3 workers")
+    say("server: staggered arrivals, batches of up to 3 per size class")
     ps = pairs()
     ref = alone(ps)
-    srv = batch.Server(n_workers=3, budget_bytes=8e9)
+    srv = batch.Server(max_batch=3, budget_bytes=16e9)
     reqs = []
     for d, p in ps:
         reqs.append(srv.submit(d, p, **KW))
@@ -77,12 +104,12 @@ def test_server_budget_serialises(hip):
     say("server: budget admits one request at a time")
     ps = pairs()[:3]
     ref = alone(ps)
-    probe = batch.Server(n_workers=1, budget_bytes=64e9)
+    probe = batch.Server(max_batch=1, budget_bytes=64e9)
     fp = [probe.submit(d, p, **KW) for d, p in ps]
     probe.drain()
     need = max(rq["req"].footprint for rq in fp)
     probe.close()
-    srv = batch.Server(n_workers=3, budget_bytes=float(need))
+    srv = batch.Server(max_batch=3, budget_bytes=float(need))
     reqs = [srv.submit(d, p, **KW) for d, p in ps]
     srv.drain()
     spans = sorted((rq["req"].t_start, rq["req"].t_done) for rq in reqs)
@@ -95,7 +122,7 @@ def test_server_budget_serialises(hip):
 
 
 def test_server_rejects_oversized_request(hip):
-    srv = batch.Server(n_workers=1, budget_bytes=1e6)
+    srv = batch.Server(max_batch=1, budget_bytes=1e6)
     d, p = synth.make_pair(128, 1, n_points=3, n_blobs=0)
     with pytest.raises(Exception, match="budget"):
         srv.submit(d, p, **KW)
