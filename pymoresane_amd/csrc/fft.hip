@@ -24,6 +24,7 @@
 // product tree from one table value.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <mutex>
 #include <vector>
 
@@ -325,64 +326,6 @@ __global__ __launch_bounds__(kThreads) void k_rows_r2c(R2cBatch bt, int q, int o
   }
 }
 
-// ---- pass 2a: column convolution.  Group g handles frequency column k:
-// forward FFT (first stage fused with the load of the n_in stored rows, zero padding implicit),
-// last forward stage + PSF product + first inverse stage fused in registers, inverse FFT with
-// its last stage fused with the store of the n_out kept rows.
-template <int R0>
-__global__ __launch_bounds__(kThreads) void k_cols_conv(ColBatch bt, int n_in, int pm, int n_out, int cpb,
-                                                        const float2* __restrict__ tw) {
-  const int jb = blockIdx.y;
-  const int* pred = bt.pred[jb];
-  if (pred && *pred == 0) return;
-  const float2* __restrict__ T1 = bt.T1[jb];
-  const float2* __restrict__ S = bt.S[jb];
-  float2* __restrict__ T2 = bt.T2[jb];
-  extern __shared__ float2 smem[];
-  const int P = 1 << pm;
-  const int ncols = (P >> 1) + 1;
-  const int nthr = kThreads / cpb;
-  const int g = threadIdx.x / nthr;
-  const int lt = threadIdx.x - g * nthr;
-  const int k = blockIdx.x * cpb + g;
-  const bool live = k < ncols;
-  float2* sm = smem + g * (P + (P >> 5));
-  const float2* col = T1 + (size_t)k * n_in;
-  const float2* sc = S + (size_t)k * P;
-  stage0_load<R0>(sm, P, lt, nthr, tw,
-                  [&](int e) { return (live && e < n_in) ? col[e] : make_float2(0.f, 0.f); });
-  __syncthreads();
-  const int ns = plan_stages(pm);
-  if (ns == 1) {
-    if (live)
-      for (int i = lt; i < P; i += nthr) sm[pad(i)] = cmul(sm[pad(i)], sc[i]);
-    __syncthreads();
-  } else {
-    mid_forward(sm, P, ns, 1, ns - 1, lt, nthr, tw);
-    for (int b = lt; b < (P >> 4); b += nthr) {
-      const int base = b << 4;
-      float2 x[16], s[16];
-#pragma unroll
-      for (int j = 0; j < 16; ++j) s[j] = live ? sc[base + j] : make_float2(0.f, 0.f);
-#pragma unroll
-      for (int j = 0; j < 16; ++j) x[j] = sm[pad(base + j)];
-      Dft<16, -1>::run(x);
-#pragma unroll
-      for (int j = 0; j < 16; ++j) x[j] = cmul(x[j], s[j]);
-      Dft<16, +1>::run(x);
-#pragma unroll
-      for (int j = 0; j < 16; ++j) sm[pad(base + j)] = x[j];
-    }
-    __syncthreads();
-    mid_inverse(sm, P, ns, 1, ns - 1, lt, nthr, tw);
-  }
-  if (!live) return;
-  float2* out = T2 + (size_t)k * n_out;
-  stage0_store<R0>(sm, P, lt, nthr, tw, [&](int e, float2 v) {
-    if (e < n_out) out[e] = v;
-  });
-}
-
 // ---- pass 2b: column forward FFT only (PSF spectrum), stored in the digit-reversed order
 template <int R0>
 __global__ __launch_bounds__(kThreads) void k_cols_fwd(const float2* __restrict__ T1, int pm, float2* __restrict__ S,
@@ -405,67 +348,500 @@ __global__ __launch_bounds__(kThreads) void k_cols_fwd(const float2* __restrict_
   for (int i = lt; i < P; i += nthr) out[i] = sm[pad(i)];
 }
 
-// ---- pass 3: rows c2r.  Group g handles kept row r; out[r * n_cols + c], c < n_cols.
-template <int R0>
-__global__ __launch_bounds__(kThreads) void k_rows_c2r(C2rBatch bt, int ldT, int pm, int n_rows, int rb, int n_cols,
-                                                       float scale, const float2* __restrict__ tw) {
+// =============================================================================================
+// Convolution passes with compile-time plans.  For a frame of side P = 2^PG the radix plan, every
+// LDS offset inside a butterfly and each thread's share of the work are constants, so the only
+// per-butterfly integer work is one base address.  Every global operand a thread needs (stage-0
+// inputs, in-stage twiddles, the PSF segment it multiplies with, the real-FFT post-processing
+// twiddles) is requested at kernel start, so a kernel pays one memory latency instead of one per
+// stage.  LIN marks the linear-convolution frame (P = 2N): half of every transform input is zero
+// padding, and only half of every inverse output is kept.  Storage order, twiddles and arithmetic
+// order per butterfly are those of the generic kernels above (same digit-reversed layout).
+
+constexpr int padc(int i) { return i + (i >> 5); }
+
+template <int LG>
+struct Plan {
+  static constexpr int L = 1 << LG;
+  static constexpr int NS = (LG + 3) >> 2;
+  static constexpr int LR0 = LG - 4 * (NS - 1);
+  static constexpr int R0 = 1 << LR0;
+  static constexpr int M0 = L / R0;
+  static constexpr int NBF = NS > 1 ? L / 16 : 0;  // radix-16 butterflies per stage
+  static constexpr int ls(int i) { return 1 << (4 * (NS - i)); }
+};
+
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for_down(F&& f) {  // E-1 down to B
+  if constexpr (B < E) {
+    f(std::integral_constant<int, E - 1>{});
+    static_for_down<B, E - 1>(f);
+  }
+}
+
+// This thread's in-stage twiddles for the radix-16 stages 1..NS-1 (butterflies b = lt + j*NT).
+template <int LG, int NT>
+struct StageTw {
+  using Pl = Plan<LG>;
+  static constexpr int PER = Pl::NBF > 0 ? (Pl::NBF + NT - 1) / NT : 1;
+  static constexpr int NI = Pl::NS > 1 ? Pl::NS : 1;
+  float2 w[NI][PER];
+  __device__ __forceinline__ void load(int lt, const float2* __restrict__ gtw) {
+    static_for<1, Pl::NS>([&](auto ic) {
+      constexpr int I = decltype(ic)::value;
+      constexpr int Ls = Pl::ls(I);
+      constexpr int M = Ls / 16;
+      if constexpr (M > 1) {
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+          const int b = lt + j * NT;
+          w[I][j] = (b < Pl::NBF) ? gtw[(b & (M - 1)) * (kFftMaxLen / Ls)] : make_float2(1.f, 0.f);
+        }
+      }
+    });
+  }
+};
+
+// radix-16 stage I of one transform in LDS (sm), forward (DIF, twiddles after the butterfly) or
+// inverse (the mirror: conjugate twiddles before the butterfly)
+template <int LG, int NT, int I, bool FWD>
+__device__ __forceinline__ void stage16(float2* sm, int lt, const StageTw<LG, NT>& tw) {
+  using Pl = Plan<LG>;
+  constexpr int Ls = Pl::ls(I);
+  constexpr int M = Ls / 16;
+  constexpr int LM = 4 * (Pl::NS - I - 1);
+  constexpr int PER = StageTw<LG, NT>::PER;
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int b = lt + j * NT;
+    if ((Pl::NBF % NT) != 0 && b >= Pl::NBF) break;
+    const int base = ((b >> LM) << (LM + 4)) + (b & (M - 1));
+    float2* s0 = sm + (base + (base >> 5));
+    float2 x[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) x[k] = s0[padc(k * M)];
+    if constexpr (FWD) {
+      Dft<16, -1>::run(x);
+      if constexpr (M > 1) {
+        float2 wk[16];
+        powers<16>(tw.w[I][j], wk);
+#pragma unroll
+        for (int k = 1; k < 16; ++k) x[k] = cmul(x[k], wk[k]);
+      }
+    } else {
+      if constexpr (M > 1) {
+        float2 wk[16];
+        powers<16>(cconj(tw.w[I][j]), wk);
+#pragma unroll
+        for (int k = 1; k < 16; ++k) x[k] = cmul(x[k], wk[k]);
+      }
+      Dft<16, +1>::run(x);
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) s0[padc(k * M)] = x[k];
+  }
+}
+
+// stage 0 (radix R0) inputs of this thread: butterflies n = lt + u*NT < M0, elements n + k*M0;
+// KL of the R0 inputs are non-zero (LIN: the upper half is zero padding)
+template <int LG, int NT, bool LIN>
+struct Stage0 {
+  using Pl = Plan<LG>;
+  static constexpr int U = (Pl::M0 + NT - 1) / NT;
+  static constexpr int KL = LIN ? (Pl::R0 / 2 > 0 ? Pl::R0 / 2 : 1) : Pl::R0;
+  float2 x[U][Pl::R0];
+  float2 w[U];
+  template <class Get>
+  __device__ __forceinline__ void load(int lt, const float2* __restrict__ gtw, Get get) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int n = lt + u * NT;
+      const bool ok = n < Pl::M0;
+#pragma unroll
+      for (int k = 0; k < Pl::R0; ++k) x[u][k] = (ok && k < KL) ? get(n + k * Pl::M0) : make_float2(0.f, 0.f);
+      if constexpr (Pl::M0 > 1) w[u] = ok ? gtw[n * (kFftMaxLen / Pl::L)] : make_float2(1.f, 0.f);
+    }
+  }
+  // forward stage 0 from registers into LDS
+  __device__ __forceinline__ void forward(float2* sm, int lt) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int n = lt + u * NT;
+      if ((Pl::M0 % NT) != 0 && n >= Pl::M0) break;
+      Dft<Pl::R0, -1>::run(x[u]);
+      if constexpr (Pl::M0 > 1) {
+        float2 wk[Pl::R0];
+        powers<Pl::R0>(w[u], wk);
+#pragma unroll
+        for (int k = 1; k < Pl::R0; ++k) x[u][k] = cmul(x[u][k], wk[k]);
+      }
+      float2* s0 = sm + (n + (n >> 5));
+#pragma unroll
+      for (int k = 0; k < Pl::R0; ++k) s0[padc(k * Pl::M0)] = x[u][k];
+    }
+  }
+  // inverse last stage from LDS to put(e, value), keeping the first KL outputs of each butterfly
+  template <class Put>
+  __device__ __forceinline__ void inverse(const float2* sm, int lt, Put put) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int n = lt + u * NT;
+      if ((Pl::M0 % NT) != 0 && n >= Pl::M0) break;
+      const float2* s0 = sm + (n + (n >> 5));
+      float2 y[Pl::R0];
+#pragma unroll
+      for (int k = 0; k < Pl::R0; ++k) y[k] = s0[padc(k * Pl::M0)];
+      if constexpr (Pl::M0 > 1) {
+        float2 wk[Pl::R0];
+        powers<Pl::R0>(cconj(w[u]), wk);
+#pragma unroll
+        for (int k = 1; k < Pl::R0; ++k) y[k] = cmul(y[k], wk[k]);
+      }
+      Dft<Pl::R0, +1>::run(y);
+#pragma unroll
+      for (int k = 0; k < KL; ++k) put(n + k * Pl::M0, y[k]);
+    }
+  }
+};
+
+// launch shapes, functions of the frame only
+template <int PG, bool LIN>
+struct RowCfg {
+  static constexpr int P = 1 << PG, L = P / 2, N = LIN ? L : P;  // N = rows = image side
+  static constexpr int rb() {
+    int r = 8192 / L;
+    if (r > 16) r = 16;
+    if (r < 1) r = 1;
+    while (r > 1 && N / r < 256) r >>= 1;
+    return r;
+  }
+  static constexpr int RB = rb();
+  static constexpr int T = 256;
+  static constexpr int NT = T / RB;
+  static constexpr int NPP = (RB * (L / 2 + 1) + T - 1) / T;  // real-FFT pairs per thread
+  static constexpr size_t LDS = (size_t)RB * (L + (L >> 5)) * sizeof(float2);
+};
+
+template <int PG>
+struct ColCfg {
+  static constexpr int P = 1 << PG;
+  static constexpr int cpb() {
+    int c = 8192 / P;
+    if (c > 16) c = 16;
+    if (c < 1) c = 1;
+    while (c > 1 && (P / 2 + 1) / c < 256) c >>= 1;
+    return c;
+  }
+  static constexpr int CPB = cpb();
+  static constexpr int NT = (P / (256 / CPB) > 32) ? P / 32 : 256 / CPB;
+  static constexpr int T = CPB * NT;
+  static constexpr size_t LDS = (size_t)CPB * (P + (P >> 5)) * sizeof(float2);
+};
+
+// storage position of frequency k (compile-time plan of length 2^LG)
+template <int LG>
+__device__ __forceinline__ int pos_of(int k) {
+  using Pl = Plan<LG>;
+  int mM = LG - Pl::LR0;
+  int pos = (k & (Pl::R0 - 1)) << mM;
+  k >>= Pl::LR0;
+#pragma unroll
+  for (int i = 1; i < Pl::NS; ++i) {
+    mM -= 4;
+    pos += (k & 15) << mM;
+    k >>= 4;
+  }
+  return pos;
+}
+
+// ---- pass 1: rows r2c.  Block = RB frame rows; output T[k*ldT + row], k in [0, P/2].
+template <int PG, bool LIN>
+__global__ __launch_bounds__(256) void k_conv_r2c(R2cBatch bt, int n_rows, int ldT, const float2* __restrict__ tw) {
+  using C = RowCfg<PG, LIN>;
+  constexpr int LG = PG - 1;
+  constexpr int L = C::L, RB = C::RB, NT = C::NT, SP = L + (L >> 5);
+  const int jb = blockIdx.y;
+  const int* pred = bt.pred[jb];
+  if (pred && *pred == 0) return;
+  const float* __restrict__ src = bt.src[jb];
+  float2* __restrict__ T = bt.T[jb];
+  extern __shared__ float2 smem[];
+  const int g = threadIdx.x / NT;
+  const int lt = threadIdx.x - g * NT;
+  const int row0 = xcd_remap(blockIdx.x, gridDim.x) * RB;
+  const int fr = row0 + g;
+  const bool live = fr < n_rows;
+  float2* sm = smem + g * SP;
+  // everything global this thread needs, requested up front
+  const float* row = src + (size_t)(live ? fr : 0) * C::N;
+  Stage0<LG, NT, LIN> s0;
+  s0.load(lt, tw, [&](int e) { return live ? *reinterpret_cast<const float2*>(row + 2 * e) : make_float2(0.f, 0.f); });
+  StageTw<LG, NT> stw;
+  stw.load(lt, tw);
+  constexpr int step = kFftMaxLen / (2 * L);
+  float2 wpp[C::NPP];
+#pragma unroll
+  for (int i = 0; i < C::NPP; ++i) {
+    const int idx = threadIdx.x + i * C::T;
+    const int k = idx / RB;
+    wpp[i] = (k <= L / 2) ? tw[k * step] : make_float2(1.f, 0.f);
+  }
+  s0.forward(sm, lt);
+  __syncthreads();
+  static_for<1, Plan<LG>::NS>([&](auto ic) {
+    stage16<LG, NT, decltype(ic)::value, true>(sm, lt, stw);
+    __syncthreads();
+  });
+  // Hermitian split of the packed transform: X[k], X[L-k] from Z[k], Z[L-k]; consecutive threads
+  // take consecutive rows of one k so the transposed writes pair up
+#pragma unroll
+  for (int i = 0; i < C::NPP; ++i) {
+    const int idx = threadIdx.x + i * C::T;
+    const int k = idx / RB;
+    const int g2 = idx - k * RB;
+    const int r2 = row0 + g2;
+    if (k > L / 2 || r2 >= n_rows) continue;
+    const float2* sm2 = smem + g2 * SP;
+    const int pk = pos_of<LG>(k & (L - 1)), pl = pos_of<LG>((L - k) & (L - 1));
+    const float2 zk = sm2[pk + (pk >> 5)];
+    const float2 zl = sm2[pl + (pl >> 5)];
+    const float2 w = wpp[i];
+    {
+      const float2 c = cconj(zl);
+      const float2 e = cscale(cadd(zk, c), 0.5f);
+      const float2 dd = cscale(csub(zk, c), 0.5f);
+      T[(size_t)k * ldT + r2] = cadd(e, cmul(w, make_float2(dd.y, -dd.x)));
+    }
+    if (k != L - k) {
+      const float2 c = cconj(zk);
+      const float2 e = cscale(cadd(zl, c), 0.5f);
+      const float2 dd = cscale(csub(zl, c), 0.5f);
+      const float2 wl = make_float2(-w.x, w.y);  // w^(L-k) = -conj(w^k)
+      T[(size_t)(L - k) * ldT + r2] = cadd(e, cmul(wl, make_float2(dd.y, -dd.x)));
+    }
+  }
+}
+
+// ---- pass 2: column convolution: forward FFT, product with the PSF column, inverse FFT.
+template <int PG, bool LIN>
+__global__ __launch_bounds__(ColCfg<PG>::T) void k_conv_cols(ColBatch bt, int n_rows, const float2* __restrict__ tw) {
+  using C = ColCfg<PG>;
+  using Pl = Plan<PG>;
+  constexpr int P = C::P, NT = C::NT, SP = P + (P >> 5);
+  constexpr int NCOLS = P / 2 + 1;
+  const int jb = blockIdx.y;
+  const int* pred = bt.pred[jb];
+  if (pred && *pred == 0) return;
+  extern __shared__ float2 smem[];
+  const int g = threadIdx.x / NT;
+  const int lt = threadIdx.x - g * NT;
+  const int k = blockIdx.x * C::CPB + g;
+  const bool live = k < NCOLS;
+  float2* sm = smem + g * SP;
+  const float2* __restrict__ col = bt.T1[jb] + (size_t)(live ? k : 0) * n_rows;
+  const float2* __restrict__ sc = bt.S[jb] + (size_t)(live ? k : 0) * P;
+  float2* __restrict__ out = bt.T2[jb] + (size_t)(live ? k : 0) * n_rows;
+  if constexpr (Pl::NS == 1) {
+    // whole transform in one thread's registers (P <= 16)
+    if (!live || lt != 0) return;
+    float2 x[P];
+#pragma unroll
+    for (int e = 0; e < P; ++e) x[e] = (!LIN || e < P / 2) ? col[e] : make_float2(0.f, 0.f);
+    Dft<P, -1>::run(x);
+#pragma unroll
+    for (int e = 0; e < P; ++e) x[e] = cmul(x[e], sc[e]);
+    Dft<P, +1>::run(x);
+#pragma unroll
+    for (int e = 0; e < (LIN ? P / 2 : P); ++e) out[e] = x[e];
+  } else {
+    Stage0<PG, NT, LIN> s0;
+    s0.load(lt, tw, [&](int e) { return live ? col[e] : make_float2(0.f, 0.f); });
+    StageTw<PG, NT> stw;
+    stw.load(lt, tw);
+    constexpr int PER = StageTw<PG, NT>::PER;
+    float2 sv[PER][16];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int b = lt + j * NT;
+#pragma unroll
+      for (int m = 0; m < 16; ++m) sv[j][m] = (live && b < Pl::NBF) ? sc[16 * b + m] : make_float2(0.f, 0.f);
+    }
+    s0.forward(sm, lt);
+    __syncthreads();
+    static_for<1, Pl::NS - 1>([&](auto ic) {
+      stage16<PG, NT, decltype(ic)::value, true>(sm, lt, stw);
+      __syncthreads();
+    });
+    // last forward stage (sub-length 16), PSF product, first inverse stage, in registers
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int b = lt + j * NT;
+      if ((Pl::NBF % NT) != 0 && b >= Pl::NBF) break;
+      const int base = 16 * b;
+      float2* s1 = sm + (base + (base >> 5));
+      float2 x[16];
+#pragma unroll
+      for (int m = 0; m < 16; ++m) x[m] = s1[m];
+      Dft<16, -1>::run(x);
+#pragma unroll
+      for (int m = 0; m < 16; ++m) x[m] = cmul(x[m], sv[j][m]);
+      Dft<16, +1>::run(x);
+#pragma unroll
+      for (int m = 0; m < 16; ++m) s1[m] = x[m];
+    }
+    __syncthreads();
+    static_for_down<1, Pl::NS - 1>([&](auto ic) {
+      stage16<PG, NT, decltype(ic)::value, false>(sm, lt, stw);
+      __syncthreads();
+    });
+    if (!live) return;
+    s0.inverse(sm, lt, [&](int e, float2 v) { out[e] = v; });
+  }
+}
+
+// ---- pass 3: rows c2r.  Block = RB kept rows; out[r * N + c].
+template <int PG, bool LIN>
+__global__ __launch_bounds__(256) void k_conv_c2r(C2rBatch bt, int n_rows, int ldT, float scale,
+                                                  const float2* __restrict__ tw) {
+  using C = RowCfg<PG, LIN>;
+  constexpr int LG = PG - 1;
+  constexpr int L = C::L, RB = C::RB, NT = C::NT, SP = L + (L >> 5);
   const int jb = blockIdx.y;
   const int* pred = bt.pred[jb];
   if (pred && *pred == 0) return;
   const float2* __restrict__ T2 = bt.T2[jb];
   float* __restrict__ out = bt.out[jb];
   extern __shared__ float2 smem[];
-  const int P = 1 << pm;
-  const int L = P >> 1;
-  const int mL = pm - 1;
-  const int nthr = kThreads / rb;
-  const int g = threadIdx.x / nthr;
-  const int lt = threadIdx.x - g * nthr;
-  const int r = xcd_remap(blockIdx.x, gridDim.x) * rb + g;
+  const int g = threadIdx.x / NT;
+  const int lt = threadIdx.x - g * NT;
+  const int row0 = xcd_remap(blockIdx.x, gridDim.x) * RB;
+  const int r = row0 + g;
   const bool live = r < n_rows;
-  float2* sm = smem + g * (L + (L >> 5));
-  const int step = kFftMaxLen / P;
-  const int half = L >> 1;
-  constexpr int U = 8;
-  for (int k0 = lt; k0 <= half; k0 += nthr * U) {
-    float2 xk[U], xl[U];
+  float2* sm = smem + g * SP;
+  constexpr int step = kFftMaxLen / (2 * L);
+  float2 xk[C::NPP], xl[C::NPP], wpp[C::NPP];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int k = k0 + u * nthr;
-      const bool ok = live && k <= half;
-      xk[u] = ok ? T2[(size_t)k * ldT + r] : make_float2(0.f, 0.f);
-      xl[u] = ok ? T2[(size_t)(L - k) * ldT + r] : make_float2(0.f, 0.f);
+  for (int i = 0; i < C::NPP; ++i) {
+    const int idx = threadIdx.x + i * C::T;
+    const int k = idx / RB;
+    const int r2 = row0 + (idx - k * RB);
+    const bool ok = k <= L / 2 && r2 < n_rows;
+    xk[i] = ok ? T2[(size_t)k * ldT + r2] : make_float2(0.f, 0.f);
+    xl[i] = ok ? T2[(size_t)(L - k) * ldT + r2] : make_float2(0.f, 0.f);
+    wpp[i] = ok ? tw[k * step] : make_float2(1.f, 0.f);
+  }
+  StageTw<LG, NT> stw;
+  stw.load(lt, tw);
+  Stage0<LG, NT, LIN> s0;
+  if constexpr (Plan<LG>::M0 > 1) {
+#pragma unroll
+    for (int u = 0; u < Stage0<LG, NT, LIN>::U; ++u) {
+      const int n = lt + u * NT;
+      s0.w[u] = (n < Plan<LG>::M0) ? tw[n * (kFftMaxLen / L)] : make_float2(1.f, 0.f);
     }
+  }
+  // Hermitian pre-twiddle: Z'[k] = (X[k] + conj X[L-k]) + i (X[k] - conj X[L-k]) conj(w^k), stored
+  // digit-reversed for the inverse transform
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int k = k0 + u * nthr;
-      if (k > half) break;
-      {
-        // Z'[k] = (X[k] + conj X[L-k]) + i (X[k] - conj X[L-k]) conj(w^k)
-        const float2 e = cadd(xk[u], cconj(xl[u]));
-        const float2 o = cmul(csub(xk[u], cconj(xl[u])), cconj(tw[k * step]));
-        sm[pad(perm_pos(k & (L - 1), mL))] = make_float2(e.x - o.y, e.y + o.x);
-      }
-      const int kl = L - k;
-      if (kl != k && kl < L) {
-        const float2 e = cadd(xl[u], cconj(xk[u]));
-        const float2 o = cmul(csub(xl[u], cconj(xk[u])), cconj(tw[kl * step]));
-        sm[pad(perm_pos(kl, mL))] = make_float2(e.x - o.y, e.y + o.x);
-      }
+  for (int i = 0; i < C::NPP; ++i) {
+    const int idx = threadIdx.x + i * C::T;
+    const int k = idx / RB;
+    const int g2 = idx - k * RB;
+    if (k > L / 2 || row0 + g2 >= n_rows) continue;
+    float2* sm2 = smem + g2 * SP;
+    const float2 w = wpp[i];
+    {
+      const float2 e = cadd(xk[i], cconj(xl[i]));
+      const float2 o = cmul(csub(xk[i], cconj(xl[i])), cconj(w));
+      const int pk = pos_of<LG>(k & (L - 1));
+      sm2[pk + (pk >> 5)] = make_float2(e.x - o.y, e.y + o.x);
+    }
+    const int kl = L - k;
+    if (kl != k && kl < L) {
+      const float2 e = cadd(xl[i], cconj(xk[i]));
+      const float2 o = cmul(csub(xl[i], cconj(xk[i])), make_float2(-w.x, -w.y));  // conj(w^(L-k)) = -w^k
+      const int pl = pos_of<LG>(kl);
+      sm2[pl + (pl >> 5)] = make_float2(e.x - o.y, e.y + o.x);
     }
   }
   __syncthreads();
-  mid_inverse(sm, L, plan_stages(mL), 1, plan_stages(mL), lt, nthr, tw);
-  if (!live) return;
-  float* orow = out + (size_t)r * n_cols;
-  stage0_store<R0>(sm, L, lt, nthr, tw, [&](int e, float2 v) {
-    const int c = 2 * e;
-    if (c + 1 < n_cols) {
-      *reinterpret_cast<float2*>(orow + c) = make_float2(v.x * scale, v.y * scale);
-    } else if (c < n_cols) {
-      orow[c] = v.x * scale;
-    }
+  static_for_down<1, Plan<LG>::NS>([&](auto ic) {
+    stage16<LG, NT, decltype(ic)::value, false>(sm, lt, stw);
+    __syncthreads();
   });
+  if (!live) return;
+  float* orow = out + (size_t)r * C::N;
+  s0.inverse(sm, lt, [&](int e, float2 v) {
+    *reinterpret_cast<float2*>(orow + 2 * e) = make_float2(v.x * scale, v.y * scale);
+  });
+}
+
+template <int PG, bool LIN>
+void set_conv_attrs(int lds_max) {
+  (void)hipFuncSetAttribute((const void*)k_conv_r2c<PG, LIN>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+  (void)hipFuncSetAttribute((const void*)k_conv_cols<PG, LIN>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+  (void)hipFuncSetAttribute((const void*)k_conv_c2r<PG, LIN>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+}
+
+template <int PG, bool LIN>
+void launch_conv(const R2cBatch& rb, const ColBatch& cb, const C2rBatch& ob, int nb, int n, float scale,
+                 const float2* tw, hipStream_t st) {
+  using R = RowCfg<PG, LIN>;
+  using C = ColCfg<PG>;
+  hipLaunchKernelGGL((k_conv_r2c<PG, LIN>), dim3((n + R::RB - 1) / R::RB, nb), dim3(R::T), R::LDS, st, rb, n, n, tw);
+  PMS_LAUNCH_CHECK();
+  hipLaunchKernelGGL((k_conv_cols<PG, LIN>), dim3((C::P / 2 + 1 + C::CPB - 1) / C::CPB, nb), dim3(C::T), C::LDS, st,
+                     cb, n, tw);
+  PMS_LAUNCH_CHECK();
+  hipLaunchKernelGGL((k_conv_c2r<PG, LIN>), dim3((n + R::RB - 1) / R::RB, nb), dim3(R::T), R::LDS, st, ob, n, n,
+                     scale, tw);
+  PMS_LAUNCH_CHECK();
+}
+
+template <bool LIN>
+void dispatch_conv(int pg, const R2cBatch& rb, const ColBatch& cb, const C2rBatch& ob, int nb, int n, float scale,
+                   const float2* tw, hipStream_t st) {
+  switch (pg) {
+    case 2: launch_conv<2, LIN>(rb, cb, ob, nb, n, scale, tw, st); break;
+    case 3: launch_conv<3, LIN>(rb, cb, ob, nb, n, scale, tw, st); break;
+    case 4: launch_conv<4, LIN>(rb, cb, ob, nb, n, scale, tw, st); break;
+    case 5: launch_conv<5, LIN>(rb, cb, ob, nb, n, scale, tw, st); break;
+    case 6: launch_conv<6, LIN>(rb, cb, ob, nb, n, scale, tw, st); break;
+    case 7: launch_conv<7, LIN>(rb, cb, ob, nb, n, scale, tw, st); break;
+    case 8: launch_conv<8, LIN>(rb, cb, ob, nb, n, scale, tw, st); break;
+    case 9: launch_conv<9, LIN>(rb, cb, ob, nb, n, scale, tw, st); break;
+    case 10: launch_conv<10, LIN>(rb, cb, ob, nb, n, scale, tw, st); break;
+    case 11: launch_conv<11, LIN>(rb, cb, ob, nb, n, scale, tw, st); break;
+    case 12: launch_conv<12, LIN>(rb, cb, ob, nb, n, scale, tw, st); break;
+    case 13: launch_conv<13, LIN>(rb, cb, ob, nb, n, scale, tw, st); break;
+    case 14: launch_conv<14, LIN>(rb, cb, ob, nb, n, scale, tw, st); break;
+    default: fail(PMS_ERR_ARG, "FFT frame out of range");
+  }
+}
+
+template <bool LIN>
+void set_all_conv_attrs(int lds_max) {
+  set_conv_attrs<2, LIN>(lds_max);
+  set_conv_attrs<3, LIN>(lds_max);
+  set_conv_attrs<4, LIN>(lds_max);
+  set_conv_attrs<5, LIN>(lds_max);
+  set_conv_attrs<6, LIN>(lds_max);
+  set_conv_attrs<7, LIN>(lds_max);
+  set_conv_attrs<8, LIN>(lds_max);
+  set_conv_attrs<9, LIN>(lds_max);
+  set_conv_attrs<10, LIN>(lds_max);
+  set_conv_attrs<11, LIN>(lds_max);
+  set_conv_attrs<12, LIN>(lds_max);
+  set_conv_attrs<13, LIN>(lds_max);
+  set_conv_attrs<14, LIN>(lds_max);
 }
 
 struct DeviceTwiddles {
@@ -502,9 +878,7 @@ int cols_per_block(int P) {
 template <int R0>
 void set_lds_attrs(int lds_max) {
   (void)hipFuncSetAttribute((const void*)k_rows_r2c<R0>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
-  (void)hipFuncSetAttribute((const void*)k_cols_conv<R0>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
   (void)hipFuncSetAttribute((const void*)k_cols_fwd<R0>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
-  (void)hipFuncSetAttribute((const void*)k_rows_c2r<R0>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
 }
 
 // first-stage radix of the plan for length 2^m
@@ -528,15 +902,6 @@ void launch_rows_r2c(const R2cBatch& bt, int count, int q, int off, int pm, int 
   PMS_LAUNCH_CHECK();
 }
 
-void launch_rows_c2r(const C2rBatch& bt, int count, int ldT, int pm, int n_rows, int n_cols, float scale,
-                     const float2* tw, hipStream_t st) {
-  const int L = (1 << pm) / 2;
-  const int rb = rows_per_block(L, n_rows);
-  const size_t lds = (size_t)rb * (L + (L >> 5)) * sizeof(float2);
-  PMS_FFT_DISPATCH(first_radix(pm - 1), k_rows_c2r, dim3(ceil_div(n_rows, rb), count), dim3(kThreads), lds, st, bt,
-                   ldT, pm, n_rows, rb, n_cols, scale, tw);
-  PMS_LAUNCH_CHECK();
-}
 
 }  // namespace
 
@@ -555,6 +920,8 @@ void fft_init_device(int device) {
   set_lds_attrs<4>(lds_max);
   set_lds_attrs<8>(lds_max);
   set_lds_attrs<16>(lds_max);
+  set_all_conv_attrs<true>(lds_max);
+  set_all_conv_attrs<false>(lds_max);
   (void)hipGetLastError();
   float2* d = nullptr;
   PMS_HIP(hipMalloc(&d, sizeof(float2) * kFftMaxLen));
@@ -617,8 +984,6 @@ void fft_conv_batch(const FftConvJob* jobs, int count, int n, int p, hipStream_t
   const float2* tw = fft_twiddles();
   const int pm = ilog2(p);
   const int L = p / 2;
-  const int cpb = cols_per_block(p);
-  const size_t lds2 = (size_t)cpb * (p + (p >> 5)) * sizeof(float2);
   const float scale = 1.0f / ((float)p * (float)p);
   for (int j0 = 0; j0 < count; j0 += kMaxBatch) {
     const int nb = std::min(kMaxBatch, count - j0);
@@ -641,11 +1006,10 @@ void fft_conv_batch(const FftConvJob* jobs, int count, int n, int p, hipStream_t
       ob.out[k] = j.out;
       ob.pred[k] = j.pred;
     }
-    launch_rows_r2c(rb, nb, n, 0, pm, n, n, tw, st);
-    PMS_FFT_DISPATCH(first_radix(pm), k_cols_conv, dim3(ceil_div(L + 1, cpb), nb), dim3(kThreads), lds2, st, cb, n,
-                     pm, n, cpb, tw);
-    PMS_LAUNCH_CHECK();
-    launch_rows_c2r(ob, nb, n, pm, n, n, scale, tw, st);
+    if (p == 2 * n)
+      dispatch_conv<true>(pm, rb, cb, ob, nb, n, scale, tw, st);
+    else
+      dispatch_conv<false>(pm, rb, cb, ob, nb, n, scale, tw, st);
   }
 }
 
