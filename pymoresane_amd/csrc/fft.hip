@@ -513,19 +513,21 @@ struct Stage0 {
 };
 
 // launch shapes, functions of the frame only
+// Rows: NT threads per row (at most 32 points each, at least one wave), RB rows per block as many as
+// 512 threads, ~140 KB of LDS and >= 256 blocks allow: more rows per block make longer contiguous
+// runs in the transposed (column-major) writes and reads.
 template <int PG, bool LIN>
 struct RowCfg {
   static constexpr int P = 1 << PG, L = P / 2, N = LIN ? L : P;  // N = rows = image side
+  static constexpr int NT = (L / 32 > 64) ? L / 32 : 64;
   static constexpr int rb() {
-    int r = 8192 / L;
-    if (r > 16) r = 16;
-    if (r < 1) r = 1;
-    while (r > 1 && N / r < 256) r >>= 1;
+    int r = 16;
+    while (r > 1 && (r * NT > 512 || (size_t)r * (L + (L >> 5)) * sizeof(float2) > 140 * 1024 || N / r < 256))
+      r >>= 1;
     return r;
   }
   static constexpr int RB = rb();
-  static constexpr int T = 256;
-  static constexpr int NT = T / RB;
+  static constexpr int T = RB * NT;
   static constexpr int NPP = (RB * (L / 2 + 1) + T - 1) / T;  // real-FFT pairs per thread
   static constexpr size_t LDS = (size_t)RB * (L + (L >> 5)) * sizeof(float2);
 };
@@ -564,7 +566,7 @@ __device__ __forceinline__ int pos_of(int k) {
 
 // ---- pass 1: rows r2c.  Block = RB frame rows; output T[k*ldT + row], k in [0, P/2].
 template <int PG, bool LIN>
-__global__ __launch_bounds__(256) void k_conv_r2c(R2cBatch bt, int n_rows, int ldT, const float2* __restrict__ tw) {
+__global__ __launch_bounds__((RowCfg<PG, LIN>::T)) void k_conv_r2c(R2cBatch bt, int n_rows, int ldT, const float2* __restrict__ tw) {
   using C = RowCfg<PG, LIN>;
   constexpr int LG = PG - 1;
   constexpr int L = C::L, RB = C::RB, NT = C::NT, SP = L + (L >> 5);
@@ -709,7 +711,7 @@ __global__ __launch_bounds__(ColCfg<PG>::T) void k_conv_cols(ColBatch bt, int n_
 
 // ---- pass 3: rows c2r.  Block = RB kept rows; out[r * N + c].
 template <int PG, bool LIN>
-__global__ __launch_bounds__(256) void k_conv_c2r(C2rBatch bt, int n_rows, int ldT, float scale,
+__global__ __launch_bounds__((RowCfg<PG, LIN>::T)) void k_conv_c2r(C2rBatch bt, int n_rows, int ldT, float scale,
                                                   const float2* __restrict__ tw) {
   using C = RowCfg<PG, LIN>;
   constexpr int LG = PG - 1;
