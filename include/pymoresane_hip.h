@@ -178,9 +178,11 @@ int pms_moresane_by_scale(const pms_moresane_params* prm, int start_scale, int s
 
 /* ------------------------------------------------------------- batched form */
 /* Independent deconvolution jobs run together.  Jobs are grouped into size
- * classes (image side, PSF side, subregion, convolution mode); each of up to
- * n_workers host threads takes a class and steps up to max_batch of its jobs
- * in lockstep on its own stream, refilling slots as jobs finish: per step one
+ * classes (image side, PSF side, subregion, convolution mode), each class
+ * split into n_workers / n_classes groups when workers outnumber classes;
+ * each of up to n_workers host threads takes a group and steps up to
+ * max_batch of its jobs in lockstep on its own stream, refilling slots as
+ * jobs finish: per step one
  * launch per kernel serves every job's CG iteration, followed by one copy of
  * the jobs' decision scalars and one synchronisation.  max_batch = 1 with one
  * worker is the plain one-job-at-a-time path.  Each job's result is

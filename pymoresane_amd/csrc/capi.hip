@@ -70,6 +70,7 @@ static Runner& runner_for(hipStream_t st) {
 // Run one task to completion on a single-slot runner; the task's error (if any) is rethrown after
 // its trace has been copied out.
 static void run_single(Runner& rn, const TaskSpec& spec, Trace& tr, int* complete, int* major) {
+  rn.refresh_options();
   const int slot = rn.add(spec);
   PMS_REQUIRE(slot >= 0, "runner busy");
   try {
@@ -400,8 +401,20 @@ int pms_batch_run(pms_batch_job* jobs, int n_jobs, int n_workers, int max_batch)
       const int sub = (p.subregion <= 0 || p.subregion > p.n) ? p.n : p.subregion;
       by_class[std::make_tuple(p.n, p.psf_n, sub, p.conv_mode)].push_back(i);
     }
+    // With more workers than classes each class is dealt round-robin into several groups, so two or
+    // more runners (streams) step the same class and the GPU has queued work while a runner's host
+    // thread turns its step around.
+    const int n_classes = (int)by_class.size();
+    const int split = std::max(1, n_workers / std::max(1, n_classes));
     std::vector<std::vector<int>> classes;
-    for (auto& kv : by_class) classes.push_back(kv.second);
+    for (auto& kv : by_class) {
+      const int parts = std::min<int>(split, (int)kv.second.size());
+      for (int q = 0; q < parts; ++q) {
+        std::vector<int> part;
+        for (size_t i = q; i < kv.second.size(); i += parts) part.push_back(kv.second[i]);
+        classes.push_back(part);
+      }
+    }
     std::stable_sort(classes.begin(), classes.end(), [&](const std::vector<int>& a, const std::vector<int>& b) {
       const double wa = (double)jobs[a[0]].prm.n * jobs[a[0]].prm.n * a.size();
       const double wb = (double)jobs[b[0]].prm.n * jobs[b[0]].prm.n * b.size();

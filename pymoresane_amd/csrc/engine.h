@@ -211,6 +211,7 @@ class Runner {
   Task* task(int slot) { return tasks_[slot].get(); }
   void release(int slot);  // drop a finished task (its workspace is kept)
   bool idle() const;
+  void refresh_options();      // re-read PMS_CG_GRAPH (opt-in graph replay of CG iterations)
   void step();                 // one lockstep step (no-op when idle)
   void run();                  // step until every task is done
   hipStream_t stream() const { return st_; }

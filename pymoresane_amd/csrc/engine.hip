@@ -840,9 +840,15 @@ Runner::Runner(int slots, hipStream_t st) : st_(st) {
   fill_u32((uint32_t*)iflag_dev_, (size_t)kSlotInts * slots, 0u, st_);  // flags and finish counters start at 0
   PMS_HIP(hipHostMalloc((void**)&scal_host_, sizeof(double) * kSlotDoubles * slots, hipHostMallocDefault));
   PMS_HIP(hipHostMalloc((void**)&upload_host_, sizeof(double) * kSlotDoubles * slots, hipHostMallocDefault));
+  refresh_options();
+  PMS_HIP(hipStreamSynchronize(st_));
+}
+
+// Replaying a captured graph of the iterations is bit-identical to eager launches but measured slower
+// for large frames on ROCm 7, so it is opt-in (PMS_CG_GRAPH=1).
+void Runner::refresh_options() {
   const char* graph_env = getenv("PMS_CG_GRAPH");
   use_graph_ = graph_env && graph_env[0] == '1';
-  PMS_HIP(hipStreamSynchronize(st_));
 }
 
 Runner::~Runner() {

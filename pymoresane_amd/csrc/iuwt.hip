@@ -304,6 +304,196 @@ __global__ __launch_bounds__(kThreads) void k_iuwt_small(ScaleParams sp) {
   }
 }
 
+// Which optional operands a walk launch has (compile-time, so the step body is straight-line).
+enum WalkFlags { F_OUTC = 1, F_OUTW = 2, F_MASK = 4, F_REF = 8 };
+
+// Output slots of steps that produce no pixel (the 2H lead-in rows, padding steps, dead lanes):
+// their stores land here instead of being branched around, which keeps every step's sequence
+// of memory operations identical.
+__device__ float g_walk_sink[kThreads];
+
+// Row walk with group pipelining.  Same comb/phase decomposition, LDS row layout and arithmetic
+// as k_iuwt_scale; the walk is cut into groups of K rows: while a group's K steps run, the input
+// rows and per-output operands (mask, reference, detail plane) of the next group are already in
+// flight, so the loop body never drains the memory pipe.  Loads never branch (dead slots read a
+// valid column) and stores never branch (no-output steps write to g_walk_sink).
+template <int MODE, int FL, int K, int ML>
+__global__ __launch_bounds__(kThreads) void k_iuwt_walk(ScaleParams sp) {
+  constexpr bool OUTC = (MODE != IUWT_DECOMP) || (FL & F_OUTC);
+  constexpr bool OUTW = (MODE == IUWT_DECOMP) && (FL & F_OUTW);
+  constexpr bool MASK = (FL & F_MASK) != 0;
+  constexpr bool REF = (MODE == IUWT_DECOMP) && (FL & F_REF);
+  constexpr bool WIN = (MODE == IUWT_RECOMP);
+  constexpr int H = ModeTraits<MODE>::H;
+  constexpr int WA = (MODE == IUWT_DECOMP) ? 7 : 5;
+  static_assert(K % 2 == 0, "group length must be even (LDS double buffer parity)");
+  const ScaleJob a = sp.job[blockIdx.y];
+  if (a.pred && *a.pred == 0) return;
+
+  extern __shared__ float lds[];
+  const int Q = 1 << sp.q_log2;
+  const int TJ = sp.tj;
+  const int row_elems = (TJ + 2 * H) * Q;
+  float* buf0 = lds;
+  float* buf1 = lds + row_elems;
+
+  const unsigned nblk = gridDim.x;
+  unsigned b = xcd_remap(blockIdx.x, nblk);
+  const int seg = b % sp.n_seg;
+  b /= sp.n_seg;
+  const int tb = b % sp.n_tb;
+  b /= sp.n_tb;
+  const int cg = b % sp.n_cg;
+  const int p = b / sp.n_cg;  // row phase
+  const int d = sp.d;
+  const int h = sp.h, w = sp.w;
+
+  const int t = threadIdx.x;
+  const int q = t & (Q - 1);
+  const int j = t >> sp.q_log2;
+  const bool thread_live = j < TJ;
+  const int col = cg * Q + q + (tb * TJ + j) * d;
+  const bool col_ok = thread_live && col < w;
+  const int col_c = col_ok ? col : 0;
+
+  const int mr = (h - p + d - 1) / d;
+  const int o_begin = seg * sp.k_rows;
+  const int o_end = min(o_begin + sp.k_rows, mr);
+  const int i_begin = o_begin - H;
+  const int n_steps = (o_end - o_begin) + 2 * H;
+  const int n_groups = (n_steps + K - 1) / K;
+
+  int ld_col[ML];
+  bool ld_ok[ML];
+#pragma unroll
+  for (int m = 0; m < ML; ++m) {
+    const int e = t + m * kThreads;
+    ld_ok[m] = e < row_elems;
+    const int jj = e >> sp.q_log2;
+    const int qq = e & (Q - 1);
+    ld_col[m] = ld_ok[m] ? reflect_idx(cg * Q + qq + (tb * TJ + jj - H) * d, w) : 0;
+  }
+  auto row_of = [&](int s) {
+    int r = p + (i_begin + s) * d;
+    if (r < 0 || r >= h) r = reflect_idx(r, h);
+    return r;
+  };
+  // output pixel of step s (clamped to a real pixel when the step has none)
+  auto out_off = [&](int s) {
+    int o = i_begin + s - H;
+    o = o < o_begin ? o_begin : (o >= o_end ? o_end - 1 : o);
+    return (size_t)(p + o * d) * w + col_c;
+  };
+  auto has_out = [&](int s) {
+    const int o = i_begin + s - H;
+    return s >= 2 * H && o < o_end && col_ok;
+  };
+
+  struct Group {
+    float row[K][ML];
+    float win[K];
+    float ref[K];
+    int msk[K];
+  };
+  auto load_group = [&](Group& G, int g) {
+#pragma unroll
+    for (int u = 0; u < K; ++u) {
+      const int s = g * K + u;
+      const float* src = a.in + (size_t)row_of(s) * w;
+#pragma unroll
+      for (int m = 0; m < ML; ++m) G.row[u][m] = src[ld_col[m]];
+      if constexpr (MASK || REF || WIN) {
+        const size_t off = out_off(s);
+        if constexpr (MASK) G.msk[u] = a.mask[off];
+        if constexpr (REF) G.ref[u] = a.ref[off];
+        if constexpr (WIN) G.win[u] = a.w_in[off];
+      }
+    }
+  };
+
+  float wa[WA];
+  float wb[9];
+  float wx[5];
+#pragma unroll
+  for (int k = 0; k < WA; ++k) wa[k] = 0.f;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) wb[k] = 0.f;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) wx[k] = 0.f;
+  double acc = 0.0;
+
+  auto step = [&](const Group& G, int u, int s) {
+    float* cur = (u & 1) ? buf1 : buf0;
+#pragma unroll
+    for (int m = 0; m < ML; ++m)
+      if (ld_ok[m]) cur[t + m * kThreads] = G.row[u][m];
+    __syncthreads();
+    const float* rowp = cur + (j + H) * Q + q;
+    float ha = 0.f, hb = 0.f, hx = 0.f;
+    if (thread_live) {
+#pragma unroll
+      for (int k = -2; k <= 2; ++k) ha = fmaf(c_h5[k + 2], rowp[k * Q], ha);
+      if (MODE == IUWT_DECOMP) {
+#pragma unroll
+        for (int k = -4; k <= 4; ++k) hb = fmaf(c_h9[k + 4], rowp[k * Q], hb);
+        hx = rowp[0];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < WA - 1; ++k) wa[k] = wa[k + 1];
+    wa[WA - 1] = ha;
+    if (MODE == IUWT_DECOMP) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) wb[k] = wb[k + 1];
+      wb[8] = hb;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) wx[k] = wx[k + 1];
+      wx[4] = hx;
+    }
+    const bool out = has_out(s);
+    const size_t off = out_off(s);
+    float c = 0.f;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) c = fmaf(c_h5[k], wa[k], c);
+    if constexpr (MODE == IUWT_DECOMP) {
+      float c1 = 0.f;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) c1 = fmaf(c_h9[k], wb[k], c1);
+      float wv = wx[0] - c1;
+      if constexpr (MASK) wv = G.msk[u] == 0 ? 0.f : wv;
+      if constexpr (OUTC) *(out ? a.out_c + off : g_walk_sink + t) = c;
+      if constexpr (OUTW) *(out ? a.out_w + off : g_walk_sink + t) = wv;
+      if constexpr (REF) {
+        const float df = out ? G.ref[u] - wv : 0.f;
+        acc = fma((double)df, (double)df, acc);
+      }
+    } else if constexpr (MODE == IUWT_SMOOTH) {
+      *(out ? a.out_c + off : g_walk_sink + t) = c;
+    } else {
+      float wi = G.win[u];
+      if constexpr (MASK) wi = G.msk[u] == 0 ? 0.f : wi;
+      *(out ? a.out_c + off : g_walk_sink + t) = c + wi;
+    }
+  };
+
+  Group GA, GB;
+  load_group(GA, 0);
+  for (int g = 0; g < n_groups; g += 2) {
+    load_group(GB, g + 1);
+#pragma unroll
+    for (int u = 0; u < K; ++u) step(GA, u, g * K + u);
+    load_group(GA, g + 2);
+    if (g + 1 >= n_groups) break;
+#pragma unroll
+    for (int u = 0; u < K; ++u) step(GB, u, (g + 1) * K + u);
+  }
+
+  if constexpr (REF) {
+    const double tot = block_sum_f64(acc);
+    if (t == 0) a.partial[blockIdx.x] = tot;
+  }
+}
+
 struct Plan {
   int q_log2, tj, k_rows, n_cg, n_tb, n_seg, blocks, lds_bytes;
 };
@@ -381,6 +571,50 @@ void launch_pf(ScaleParams& sp, int count, const Plan& pl, hipStream_t st) {
   PMS_LAUNCH_CHECK();
 }
 
+constexpr int kWalkGroup = 4;
+
+template <int MODE, int FL>
+void launch_walk_fl(ScaleParams& sp, int count, const Plan& pl, int halo, hipStream_t st) {
+  const int elems = (pl.tj + 2 * halo) << pl.q_log2;
+  dim3 grid(pl.blocks, count), block(kThreads);
+  if (elems <= 2 * kThreads)
+    hipLaunchKernelGGL((k_iuwt_walk<MODE, FL, kWalkGroup, 2>), grid, block, pl.lds_bytes, st, sp);
+  else if (elems <= 4 * kThreads)
+    hipLaunchKernelGGL((k_iuwt_walk<MODE, FL, kWalkGroup, 4>), grid, block, pl.lds_bytes, st, sp);
+  else
+    hipLaunchKernelGGL((k_iuwt_walk<MODE, FL, kWalkGroup, 8>), grid, block, pl.lds_bytes, st, sp);
+  PMS_LAUNCH_CHECK();
+}
+
+template <int MODE>
+void launch_walk(ScaleParams& sp, int count, const Plan& pl, int halo, int fl, hipStream_t st) {
+  if constexpr (MODE == IUWT_DECOMP) {
+    switch (fl) {
+#define PMS_WALK_CASE(F) \
+  case F: launch_walk_fl<MODE, F>(sp, count, pl, halo, st); break;
+      PMS_WALK_CASE(0) PMS_WALK_CASE(1) PMS_WALK_CASE(2) PMS_WALK_CASE(3) PMS_WALK_CASE(4) PMS_WALK_CASE(5)
+      PMS_WALK_CASE(6) PMS_WALK_CASE(7) PMS_WALK_CASE(8) PMS_WALK_CASE(9) PMS_WALK_CASE(10) PMS_WALK_CASE(11)
+      PMS_WALK_CASE(12) PMS_WALK_CASE(13) PMS_WALK_CASE(14) PMS_WALK_CASE(15)
+#undef PMS_WALK_CASE
+      default: fail(PMS_ERR_INTERNAL, "bad IUWT walk flags");
+    }
+  } else if constexpr (MODE == IUWT_RECOMP) {
+    if (fl & F_MASK)
+      launch_walk_fl<MODE, F_MASK>(sp, count, pl, halo, st);
+    else
+      launch_walk_fl<MODE, 0>(sp, count, pl, halo, st);
+  } else {
+    launch_walk_fl<MODE, 0>(sp, count, pl, halo, st);
+  }
+}
+
+int walk_flags(const IuwtScaleCall& c) {
+  if (c.mode == IUWT_DECOMP)
+    return (c.out_c ? F_OUTC : 0) | (c.out_w ? F_OUTW : 0) | (c.mask ? F_MASK : 0) | (c.ref ? F_REF : 0);
+  if (c.mode == IUWT_RECOMP) return c.mask ? F_MASK : 0;
+  return 0;
+}
+
 template <int MODE>
 void launch_mode(ScaleParams& sp, int count, const Plan& pl, int halo, hipStream_t st) {
   const int elems = (pl.tj + 2 * halo) << pl.q_log2;
@@ -447,6 +681,29 @@ int iuwt_scale_batch(const IuwtScaleCall* calls, int count, hipStream_t st) {
   sp.n_tb = pl.n_tb;
   sp.n_seg = pl.n_seg;
   int blocks = 0;
+  if (!small) {
+    // one launch per operand layout (flag set), up to kMaxBatch jobs each
+    std::vector<int> order(count);
+    for (int k = 0; k < count; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return walk_flags(calls[x]) < walk_flags(calls[y]); });
+    for (int i0 = 0; i0 < count;) {
+      const int fl = walk_flags(calls[order[i0]]);
+      int nb = 0;
+      while (i0 + nb < count && nb < kMaxBatch && walk_flags(calls[order[i0 + nb]]) == fl) {
+        sp.job[nb] = scale_job(calls[order[i0 + nb]]);
+        ++nb;
+      }
+      if (c0.mode == IUWT_DECOMP)
+        launch_walk<IUWT_DECOMP>(sp, nb, pl, halo, fl, st);
+      else if (c0.mode == IUWT_SMOOTH)
+        launch_walk<IUWT_SMOOTH>(sp, nb, pl, halo, fl, st);
+      else
+        launch_walk<IUWT_RECOMP>(sp, nb, pl, halo, fl, st);
+      i0 += nb;
+    }
+    return pl.blocks;
+  }
   for (int j0 = 0; j0 < count; j0 += kMaxBatch) {
     const int nb = std::min(kMaxBatch, count - j0);
     for (int k = 0; k < nb; ++k) sp.job[k] = scale_job(calls[j0 + k]);

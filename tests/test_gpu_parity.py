@@ -35,6 +35,8 @@ def rel(a, b):
 @pytest.mark.parametrize("shape,scales,adjust", [
     ((64, 64), 5, 0), ((64, 64), 5, 2), ((48, 80), 4, 1), ((40, 40), 4, 0), ((256, 256), 7, 0),
     ((256, 256), 4, 3), ((96, 32), 4, 0), ((512, 512), 6, 0),
+    # planes above 512^2 pixels take the row-walk kernel
+    ((1024, 1024), 5, 0), ((768, 1040), 4, 1), ((1040, 600), 6, 2),
 ])
 def test_iuwt_decomposition_matches_oracle(hip, shape, scales, adjust):
     say("iuwt decomposition %s S=%d adj=%d" % (shape, scales, adjust))
@@ -50,6 +52,7 @@ def test_iuwt_decomposition_matches_oracle(hip, shape, scales, adjust):
 @pytest.mark.parametrize("shape,scales,adjust,smoothed", [
     ((64, 64), 5, 0, True), ((64, 64), 5, 2, False), ((48, 80), 4, 1, True), ((256, 256), 6, 0, False),
     ((32, 32), 1, 0, False), ((32, 32), 4, 3, False),
+    ((768, 1040), 4, 1, True), ((1024, 1024), 5, 0, False), ((1040, 600), 3, 2, False),
 ])
 def test_iuwt_recomposition_matches_oracle(hip, shape, scales, adjust, smoothed):
     say("iuwt recomposition %s S=%d adj=%d smoothed=%s" % (shape, scales, adjust, smoothed))
@@ -258,6 +261,18 @@ def test_moresane_config_c0(hip):
               enforce_positivity=True, conv_mode="linear", minor_loop_miter=30, accuracy=1e-6)
     img, ref = run_pair(dirty, psf, **kw)
     say("c0 model rel %.3e residual rel %.3e" % (rel(img.model, ref.model), rel(img.residual, ref.residual)))
+    assert discrete(img.trace) == discrete(ref.trace)
+    assert rel(img.model, ref.model) < 1e-4
+    assert rel(img.residual, ref.residual) < 1e-4
+
+
+def test_moresane_row_walk_size(hip):
+    """1024^2: the IUWT planes take the row-walk kernel (masked/reference variants in the CG loop)."""
+    say("moresane 1024^2, S=3, 2 major iterations")
+    dirty, psf = synth.make_pair(1024, 24, n_points=30, n_blobs=3)
+    kw = dict(scale_count=3, loop_gain=0.2, major_loop_miter=2, minor_loop_miter=12, enforce_positivity=True)
+    img, ref = run_pair(dirty, psf, **kw)
+    say("1024 model rel %.3e residual rel %.3e" % (rel(img.model, ref.model), rel(img.residual, ref.residual)))
     assert discrete(img.trace) == discrete(ref.trace)
     assert rel(img.model, ref.model) < 1e-4
     assert rel(img.residual, ref.residual) < 1e-4
