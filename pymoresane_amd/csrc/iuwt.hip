@@ -14,9 +14,9 @@
 // halo of a few teeth, whatever d is.  The workgroup walks down its row
 // phase: each input row of the comb (plus halo teeth) is staged in LDS, the
 // horizontal taps are read from LDS, and the vertical taps come from
-// per-thread register windows that slide one row per step.  Input rows are
-// prefetched PF rows ahead into registers so several rows are in flight per
-// workgroup.
+// per-thread register windows that slide one row per step.  The walk runs in
+// groups of rows: while one group is processed, the next group's input rows
+// and per-output operands are already in flight (see k_iuwt_walk).
 //
 // One decomposition scale is one launch: from C0 it produces C = H C0 (5-tap
 // in both axes) and w = C0 - H(H C0), using the 9-tap filter B3*B3 for the
@@ -75,168 +75,9 @@ struct ModeTraits<IUWT_RECOMP> {
   static constexpr int H = 2;
 };
 
-template <int MODE, int PF, int ML>
-__global__ __launch_bounds__(kThreads) void k_iuwt_scale(ScaleParams sp) {
-  const ScaleJob a = sp.job[blockIdx.y];
-  constexpr int kMaxLoads = ML;  // LDS-staging loads per thread per comb row
-  constexpr int H = ModeTraits<MODE>::H;
-  constexpr int WA = (MODE == IUWT_DECOMP) ? 7 : 5;  // window of horizontal 5-tap results
-  if (a.pred && *a.pred == 0) return;
-
-  extern __shared__ float lds[];
-  const int Q = 1 << sp.q_log2;
-  const int TJ = sp.tj;
-  const int row_elems = (TJ + 2 * H) * Q;
-  float* buf0 = lds;
-  float* buf1 = lds + row_elems;
-
-  // decode the (remapped) block id: consecutive logical ids walk consecutive segments of one strip
-  const unsigned nblk = gridDim.x;
-  unsigned b = xcd_remap(blockIdx.x, nblk);
-  const int seg = b % sp.n_seg;
-  b /= sp.n_seg;
-  const int tb = b % sp.n_tb;
-  b /= sp.n_tb;
-  const int cg = b % sp.n_cg;
-  const int p = b / sp.n_cg;  // row phase
-  const int d = sp.d;
-
-  const int t = threadIdx.x;
-  const int q = t & (Q - 1);
-  const int j = t >> sp.q_log2;
-  const bool thread_live = j < TJ;
-  const int col = cg * Q + q + (tb * TJ + j) * d;
-  const bool col_ok = thread_live && col < sp.w;
-
-  // rows of this phase: r = p + i*d for i in [0, mr)
-  const int mr = (sp.h - p + d - 1) / d;
-  const int o_begin = seg * sp.k_rows;
-  const int o_end = min(o_begin + sp.k_rows, mr);
-  const int i_begin = o_begin - H;
-  const int n_steps = (o_end - o_begin) + 2 * H;
-  const int w = sp.w;
-
-  // per-thread staging slots for one comb row
-  int ld_col[kMaxLoads];
-#pragma unroll
-  for (int m = 0; m < kMaxLoads; ++m) {
-    const int e = t + m * kThreads;
-    ld_col[m] = -1;
-    if (e < row_elems) {
-      const int jj = e >> sp.q_log2;
-      const int qq = e & (Q - 1);
-      ld_col[m] = reflect_idx(cg * Q + qq + (tb * TJ + jj - H) * d, w);
-    }
-  }
-
-  float regs[PF][kMaxLoads];
-  auto load_row = [&](float (&dst)[kMaxLoads], int i) {
-    const int r = reflect_idx(p + i * d, sp.h);
-    const float* src = a.in + (size_t)r * w;
-#pragma unroll
-    for (int m = 0; m < kMaxLoads; ++m)
-      if (ld_col[m] >= 0) dst[m] = src[ld_col[m]];
-  };
-  auto store_row = [&](const float (&src)[kMaxLoads], float* dstbuf) {
-#pragma unroll
-    for (int m = 0; m < kMaxLoads; ++m)
-      if (ld_col[m] >= 0) dstbuf[t + m * kThreads] = src[m];
-  };
-
-  float wa[WA];
-  float wb[9];
-  float wx[5];
-#pragma unroll
-  for (int k = 0; k < WA; ++k) wa[k] = 0.f;
-#pragma unroll
-  for (int k = 0; k < 9; ++k) wb[k] = 0.f;
-#pragma unroll
-  for (int k = 0; k < 5; ++k) wx[k] = 0.f;
-  double acc = 0.0;
-
-  // prologue: PF rows in flight, first row staged
-#pragma unroll
-  for (int u = 0; u < PF; ++u)
-    if (u < n_steps) load_row(regs[u], i_begin + u);
-  store_row(regs[0], buf0);
-  __syncthreads();
-
-  for (int s0 = 0; s0 < n_steps; s0 += PF) {
-#pragma unroll
-    for (int u = 0; u < PF; ++u) {
-      const int s = s0 + u;
-      if (s < n_steps) {
-        const int i = i_begin + s;
-        float* cur = (s & 1) ? buf1 : buf0;
-        float* nxt = (s & 1) ? buf0 : buf1;
-        if (s + PF < n_steps) load_row(regs[u], i + PF);
-
-        // horizontal taps from LDS
-        const float* rowp = cur + (j + H) * Q + q;
-        float ha = 0.f, hb = 0.f, hx = 0.f;
-        if (thread_live) {
-#pragma unroll
-          for (int k = -2; k <= 2; ++k) ha = fmaf(c_h5[k + 2], rowp[k * Q], ha);
-          if (MODE == IUWT_DECOMP) {
-#pragma unroll
-            for (int k = -4; k <= 4; ++k) hb = fmaf(c_h9[k + 4], rowp[k * Q], hb);
-            hx = rowp[0];
-          }
-        }
-        // slide the vertical windows
-#pragma unroll
-        for (int k = 0; k < WA - 1; ++k) wa[k] = wa[k + 1];
-        wa[WA - 1] = ha;
-        if (MODE == IUWT_DECOMP) {
-#pragma unroll
-          for (int k = 0; k < 8; ++k) wb[k] = wb[k + 1];
-          wb[8] = hb;
-#pragma unroll
-          for (int k = 0; k < 4; ++k) wx[k] = wx[k + 1];
-          wx[4] = hx;
-        }
-
-        if (s >= 2 * H && col_ok) {
-          const int o = i - H;                      // output row in phase units
-          const size_t off = (size_t)(p + o * d) * w + col;
-          float c = 0.f;
-#pragma unroll
-          for (int k = 0; k < 5; ++k) c = fmaf(c_h5[k], wa[k], c);
-          if (MODE == IUWT_DECOMP) {
-            float c1 = 0.f;
-#pragma unroll
-            for (int k = 0; k < 9; ++k) c1 = fmaf(c_h9[k], wb[k], c1);
-            float wv = wx[0] - c1;
-            if (a.mask && a.mask[off] == 0) wv = 0.f;
-            if (a.out_c) a.out_c[off] = c;
-            if (a.out_w) a.out_w[off] = wv;
-            if (a.ref) {
-              const float df = a.ref[off] - wv;
-              acc = fma((double)df, (double)df, acc);
-            }
-          } else if (MODE == IUWT_SMOOTH) {
-            a.out_c[off] = c;
-          } else {
-            float wi = a.w_in[off];
-            if (a.mask && a.mask[off] == 0) wi = 0.f;
-            a.out_c[off] = c + wi;
-          }
-        }
-        if (s + 1 < n_steps) store_row(regs[(u + 1) % PF], nxt);
-        __syncthreads();
-      }
-    }
-  }
-
-  if (MODE == IUWT_DECOMP && a.ref) {
-    const double tot = block_sum_f64(acc);
-    if (t == 0) a.partial[blockIdx.x] = tot;
-  }
-}
-
 // Small planes (whole plane L2-resident): one thread per output pixel loads its full dilated
 // 9x9 (decomposition) or 5x5 neighbourhood at once, so a scale costs one memory latency instead
-// of a row walk.  Same arithmetic as k_iuwt_scale: row sums first, then the vertical combination.
+// of a row walk.  Same arithmetic as k_iuwt_walk: row sums first, then the vertical combination.
 template <int MODE>
 __global__ __launch_bounds__(kThreads) void k_iuwt_small(ScaleParams sp) {
   const ScaleJob a = sp.job[blockIdx.z];
@@ -251,11 +92,11 @@ __global__ __launch_bounds__(kThreads) void k_iuwt_small(ScaleParams sp) {
   const int row = live ? row0 : 0;
   int cidx[T];
 #pragma unroll
-  for (int b = 0; b < T; ++b) cidx[b] = reflect_idx(col + (b - H) * d, w);
+  for (int b = 0; b < T; ++b) cidx[b] = fold_idx(col + (b - H) * d, w);
   float ra[T], rb[T], xc = 0.f;
 #pragma unroll
   for (int i = 0; i < T; ++i) {
-    const float* src = a.in + (size_t)reflect_idx(row + (i - H) * d, h) * w;
+    const float* src = a.in + (size_t)fold_idx(row + (i - H) * d, h) * w;
     float v[T];
 #pragma unroll
     for (int b = 0; b < T; ++b) v[b] = src[cidx[b]];
@@ -312,8 +153,8 @@ enum WalkFlags { F_OUTC = 1, F_OUTW = 2, F_MASK = 4, F_REF = 8 };
 // of memory operations identical.
 __device__ float g_walk_sink[kThreads];
 
-// Row walk with group pipelining.  Same comb/phase decomposition, LDS row layout and arithmetic
-// as k_iuwt_scale; the walk is cut into groups of K rows: while a group's K steps run, the input
+// Row walk with group pipelining over the comb/phase decomposition described at the top of this
+// file; the walk is cut into groups of K rows: while a group's K steps run, the input
 // rows and per-output operands (mask, reference, detail plane) of the next group are already in
 // flight, so the loop body never drains the memory pipe.  Loads never branch (dead slots read a
 // valid column) and stores never branch (no-output steps write to g_walk_sink).
@@ -371,12 +212,10 @@ __global__ __launch_bounds__(kThreads) void k_iuwt_walk(ScaleParams sp) {
     ld_ok[m] = e < row_elems;
     const int jj = e >> sp.q_log2;
     const int qq = e & (Q - 1);
-    ld_col[m] = ld_ok[m] ? reflect_idx(cg * Q + qq + (tb * TJ + jj - H) * d, w) : 0;
+    ld_col[m] = ld_ok[m] ? fold_idx(cg * Q + qq + (tb * TJ + jj - H) * d, w) : 0;
   }
   auto row_of = [&](int s) {
-    int r = p + (i_begin + s) * d;
-    if (r < 0 || r >= h) r = reflect_idx(r, h);
-    return r;
+    return fold_idx(p + (i_begin + s) * d, h);
   };
   // output pixel of step s (clamped to a real pixel when the step has none)
   auto out_off = [&](int s) {
@@ -505,12 +344,11 @@ int env_int(const char* name, int dflt) {
 
 // tuning knobs, read once per process
 struct IuwtTuning {
-  int qmax, krows, pf, target_blocks;
+  int qmax, krows, target_blocks;
   long small_pixels;  // planes up to this many pixels use the direct-stencil kernel
   IuwtTuning()
       : qmax(env_int("PMS_IUWT_QMAX", 16)),
         krows(env_int("PMS_IUWT_KROWS", 0)),
-        pf(env_int("PMS_IUWT_PF", 3)),
         target_blocks(env_int("PMS_IUWT_BLOCKS", 2048)),
         small_pixels(env_int("PMS_IUWT_SMALL", 512 * 512)) {}
 };
@@ -554,24 +392,7 @@ Plan make_plan(int h, int w, int scale, int halo) {
   return pl;
 }
 
-template <int MODE, int ML>
-void launch_pf(ScaleParams& sp, int count, const Plan& pl, hipStream_t st) {
-  const int pf = tuning().pf;
-  dim3 grid(pl.blocks, count), block(kThreads);
-  if (pf <= 1)
-    hipLaunchKernelGGL((k_iuwt_scale<MODE, 1, ML>), grid, block, pl.lds_bytes, st, sp);
-  else if (pf == 2)
-    hipLaunchKernelGGL((k_iuwt_scale<MODE, 2, ML>), grid, block, pl.lds_bytes, st, sp);
-  else if (pf == 3)
-    hipLaunchKernelGGL((k_iuwt_scale<MODE, 3, ML>), grid, block, pl.lds_bytes, st, sp);
-  else if (pf <= 4)
-    hipLaunchKernelGGL((k_iuwt_scale<MODE, 4, ML>), grid, block, pl.lds_bytes, st, sp);
-  else
-    hipLaunchKernelGGL((k_iuwt_scale<MODE, 6, ML>), grid, block, pl.lds_bytes, st, sp);
-  PMS_LAUNCH_CHECK();
-}
-
-constexpr int kWalkGroup = 4;
+constexpr int kWalkGroup = 2;
 
 template <int MODE, int FL>
 void launch_walk_fl(ScaleParams& sp, int count, const Plan& pl, int halo, hipStream_t st) {
@@ -615,16 +436,6 @@ int walk_flags(const IuwtScaleCall& c) {
   return 0;
 }
 
-template <int MODE>
-void launch_mode(ScaleParams& sp, int count, const Plan& pl, int halo, hipStream_t st) {
-  const int elems = (pl.tj + 2 * halo) << pl.q_log2;
-  if (elems <= 2 * kThreads)
-    launch_pf<MODE, 2>(sp, count, pl, st);
-  else if (elems <= 4 * kThreads)
-    launch_pf<MODE, 4>(sp, count, pl, st);
-  else
-    launch_pf<MODE, 8>(sp, count, pl, st);
-}
 
 }  // namespace
 
@@ -707,26 +518,16 @@ int iuwt_scale_batch(const IuwtScaleCall* calls, int count, hipStream_t st) {
   for (int j0 = 0; j0 < count; j0 += kMaxBatch) {
     const int nb = std::min(kMaxBatch, count - j0);
     for (int k = 0; k < nb; ++k) sp.job[k] = scale_job(calls[j0 + k]);
-    if (small) {
-      const dim3 g2 = small_grid(c0.h, c0.w);
-      const dim3 grid(g2.x, g2.y, nb);
-      if (c0.mode == IUWT_DECOMP)
-        hipLaunchKernelGGL(k_iuwt_small<IUWT_DECOMP>, grid, dim3(kThreads), 0, st, sp);
-      else if (c0.mode == IUWT_SMOOTH)
-        hipLaunchKernelGGL(k_iuwt_small<IUWT_SMOOTH>, grid, dim3(kThreads), 0, st, sp);
-      else
-        hipLaunchKernelGGL(k_iuwt_small<IUWT_RECOMP>, grid, dim3(kThreads), 0, st, sp);
-      PMS_LAUNCH_CHECK();
-      blocks = (int)(g2.x * g2.y);
-    } else {
-      if (c0.mode == IUWT_DECOMP)
-        launch_mode<IUWT_DECOMP>(sp, nb, pl, halo, st);
-      else if (c0.mode == IUWT_SMOOTH)
-        launch_mode<IUWT_SMOOTH>(sp, nb, pl, halo, st);
-      else
-        launch_mode<IUWT_RECOMP>(sp, nb, pl, halo, st);
-      blocks = pl.blocks;
-    }
+    const dim3 g2 = small_grid(c0.h, c0.w);
+    const dim3 grid(g2.x, g2.y, nb);
+    if (c0.mode == IUWT_DECOMP)
+      hipLaunchKernelGGL(k_iuwt_small<IUWT_DECOMP>, grid, dim3(kThreads), 0, st, sp);
+    else if (c0.mode == IUWT_SMOOTH)
+      hipLaunchKernelGGL(k_iuwt_small<IUWT_SMOOTH>, grid, dim3(kThreads), 0, st, sp);
+    else
+      hipLaunchKernelGGL(k_iuwt_small<IUWT_RECOMP>, grid, dim3(kThreads), 0, st, sp);
+    PMS_LAUNCH_CHECK();
+    blocks = (int)(g2.x * g2.y);
   }
   return blocks;
 }

@@ -61,6 +61,14 @@ __device__ __forceinline__ int reflect_idx(int i, int n) {
   return (m >= n) ? (p - 1 - m) : m;
 }
 
+// The same extension for indices within two periods of the range, i in [-2n, 3n), without a
+// division: the a trous taps never reach further (offsets up to 4*2^s with 2^(s+1) <= n).
+__device__ __forceinline__ int fold_idx(int i, int n) {
+  i = i < 0 ? i + 2 * n : i;
+  i = i >= 2 * n ? i - 2 * n : i;
+  return i >= n ? 2 * n - 1 - i : i;
+}
+
 // XCD-aware block remap: hardware dispatches consecutive workgroups
 // round-robin over the 8 XCDs; this gives each XCD a contiguous range of
 // logical blocks so neighbouring tiles share one L2.
