@@ -42,10 +42,35 @@ def check_scale(shape, scale):
             "scale %d needs 2**(scale+1)=%d <= image side %s" % (scale, d2, tuple(shape[-2:])))
 
 
+_SCIPY_BACKEND = False
+
+
+def use_scipy_backend(flag=True):
+    """Route ``a_trous`` through ``scipy.ndimage.correlate1d`` (mode='reflect' on both axes).
+
+    That library routine is the pin of the restatement below (equal to ~1e-16, checked in
+    tests/test_oracle_pins.py) and is several times faster on large planes; long oracle runs
+    (the 2048^2 by-scale case) use it.
+    """
+    global _SCIPY_BACKEND
+    _SCIPY_BACKEND = bool(flag)
+
+
+def _a_trous_scipy(c0, scale):
+    from scipy import ndimage
+    d = 1 << scale
+    k = np.zeros(4 * d + 1)
+    k[::d] = B3
+    v = ndimage.correlate1d(c0, k, axis=0, mode="reflect")
+    return ndimage.correlate1d(v, k, axis=1, mode="reflect")
+
+
 def a_trous(c0, scale):
     """One B3 a trous smoothing at dilation 2**scale (iuwt.py:151-191)."""
     c0 = np.asarray(c0, dtype=np.float64)
     check_scale(c0.shape, scale)
+    if _SCIPY_BACKEND:
+        return _a_trous_scipy(c0, scale)
     d = 1 << scale
     h, w = c0.shape
     taps = [(k * d, B3[k + 2]) for k in range(-2, 3)]

@@ -239,3 +239,26 @@ def test_snr_ratio_closed_form():
     a = np.array([3.0, 4.0])      # |a| = 5
     b = np.array([3.0, 3.5])      # |a-b| = 0.5
     assert tools.snr_ratio(a, b) == pytest.approx(20.0)
+
+
+def test_fast_backends_match_restatement():
+    """The scipy a-trous backend and threaded transforms used for long oracle runs agree with the
+    restatement / numpy.fft to rounding level."""
+    from oracle import convolution64 as conv
+    x = np.random.default_rng(9).standard_normal((48, 80))
+    ref = [iuwt.a_trous(x, s) for s in range(4)]
+    iuwt.use_scipy_backend(True)
+    try:
+        for s in range(4):
+            assert np.max(np.abs(iuwt.a_trous(x, s) - ref[s])) < 1e-14
+    finally:
+        iuwt.use_scipy_backend(False)
+    img = np.random.default_rng(10).standard_normal((32, 32))
+    psf = np.random.default_rng(11).standard_normal((64, 64))
+    a = conv.fft_convolve(img, conv.psf_spectrum(psf), "linear")
+    conv.set_fft_workers(4)
+    try:
+        b = conv.fft_convolve(img, conv.psf_spectrum(psf), "linear")
+    finally:
+        conv.set_fft_workers(1)
+    assert np.max(np.abs(a - b)) < 1e-12
