@@ -111,6 +111,14 @@ struct TaskSpec {
 
 class Runner;
 
+// Work of one step that the runner issues as batched launches across tasks (between every task's
+// pre and post work of that step), keyed by plane side or (image side, FFT frame).
+struct SharedOps {
+  std::vector<std::pair<int, IuwtDecompJob>> decomp;
+  std::vector<std::pair<int, IuwtRecompJob>> recomp;
+  std::vector<std::pair<std::pair<int, int>, FftConvJob>> conv;
+};
+
 // Buffers and scales of one CG iteration of one job (also the key of a captured iteration graph).
 struct IterArgs {
   float *x, *xn, *r, *rn, *p, *ap;
@@ -132,7 +140,10 @@ class Task {
 
   Phase phase() const { return phase_; }
   bool done() const { return phase_ == DONE; }
-  void enqueue(hipStream_t st);                     // GPU work of the current phase (MINOR: start only)
+  // GPU work of the current phase, in three parts: pre, the batched middle (add_shared), post
+  void enqueue_pre(hipStream_t st);
+  void add_shared(SharedOps& ops);
+  void enqueue_post(hipStream_t st);
   bool wants_iteration() const { return phase_ == MINOR && iterate_; }
   IterArgs iter_args() const;
   void consume(const double* slot_host);            // host decisions after the step's sync
@@ -159,9 +170,6 @@ class Task {
   void minor_exit();
   void after_inner();
   void enqueue_setup(hipStream_t st);
-  void enqueue_decomp(hipStream_t st);
-  void enqueue_minor_start(hipStream_t st);
-  void enqueue_update(hipStream_t st, double gain);
   void enqueue_finish(hipStream_t st);
   const float* dirty_dev() const { return spec.by_scale ? job_->cur_dirty : spec.dirty; }
 
@@ -219,6 +227,7 @@ class Runner {
   size_t device_bytes() const;
 
  private:
+  void run_shared(const std::vector<int>& active);
   void run_iterations(std::vector<Task*>& ts);
   void enqueue_iterations(const std::vector<IterArgs>& its, const std::vector<Task*>& ts);
   void apply_op(const std::vector<Task*>& ts, const std::vector<IterArgs>& its, bool clip_pass);

@@ -452,22 +452,110 @@ void Task::start_call(int scale_count) {
   phase_ = SETUP;
 }
 
-void Task::enqueue(hipStream_t st) {
+// Step work by phase.  pre: everything up to the shared part; shared: the decomposition of the dirty
+// subregion (DECOMP), the recomposition of the extracted sources (first MINOR step), the full-frame
+// model convolution (UPDATE / REVERT); post: the rest.  Stream order keeps pre < shared < post.
+void Task::enqueue_pre(hipStream_t st) {
   if (upload_energies_ > 0) {
     PMS_HIP(hipMemcpyAsync(scal_ + kEnergy, upload_, sizeof(double) * upload_energies_, hipMemcpyHostToDevice, st));
     upload_energies_ = 0;
   }
   ++steps;
+  Job& J = *job_;
+  const int sub = g_.sub;
+  const size_t s2 = (size_t)sub * sub;
   switch (phase_) {
     case SETUP: enqueue_setup(st); break;
-    case DECOMP: enqueue_decomp(st); break;
-    case MINOR:
-      if (minor_first_) enqueue_minor_start(st);
+    case DECOMP:
+      if (spec.mask) {
+        // decomposition of dirty_sub * mask[sub] replaces the thresholded one (main.py:283-285)
+        iuwt_decompose(J.dirty_sub, sub, sub, g_.S, 0, J.D, nullptr, J.iuwt_work, nullptr, nullptr, nullptr, nullptr,
+                       st);
+        mad_thresholds(J.D, g_.S, sub, sub, nm_, scal_ + kThr, J.mad_work, nullptr, st);
+        window_copy(spec.mask, g_.n, g_.lo, g_.lo, sub, nullptr, J.cbuf, st);
+        window_copy(J.dirty_sub, sub, 0, 0, sub, J.cbuf, J.rec, st);
+        iuwt_decompose(J.rec, sub, sub, g_.S, 0, J.D, nullptr, J.iuwt_work, nullptr, nullptr, nullptr, nullptr, st);
+      }
       break;
-    case UPDATE: enqueue_update(st, spec.prm.loop_gain); break;
-    case REVERT: enqueue_update(st, -spec.prm.loop_gain); break;
+    case MINOR:
+      if (minor_first_) {
+        // source extraction (main.py:347-355); masks/extracted live at planes [adj, max_scale) so the
+        // decomposition can index them by scale
+        const int adj = scale_adjust_;
+        source_extract(J.D + (size_t)adj * s2, max_scale_ - adj, sub, sub, spec.prm.tolerance, spec.prm.neg_comp,
+                       J.M + (size_t)adj * s2, J.E + (size_t)adj * s2, nullptr, J.ext_work, nullptr, st);
+      }
+      break;
+    case UPDATE:
+    case REVERT: {
+      // model += gain x (a negative gain reverts the update), main.py:479-497
+      const double gain = phase_ == UPDATE ? spec.prm.loop_gain : -spec.prm.loop_gain;
+      hipLaunchKernelGGL(k_model_add, dim3(grid_for(s2)), dim3(kT), 0, st, J.model_local, g_.n, g_.lo, g_.lo, sub,
+                         (const float*)x_, gain);
+      PMS_LAUNCH_CHECK();
+      break;
+    }
     case FINISH: enqueue_finish(st); break;
     case DONE: break;
+  }
+}
+
+void Task::add_shared(SharedOps& ops) {
+  Job& J = *job_;
+  const int sub = g_.sub;
+  const size_t s2 = (size_t)sub * sub;
+  if (phase_ == DECOMP && !spec.mask) {
+    ops.decomp.push_back({sub, IuwtDecompJob{J.dirty_sub, g_.S, 0, J.D, nullptr, J.iuwt_work, nullptr, nullptr,
+                                             nullptr, nullptr, 0}});
+  } else if (phase_ == MINOR && minor_first_) {
+    const int adj = scale_adjust_;
+    ops.recomp.push_back({sub, IuwtRecompJob{J.E + (size_t)adj * s2, max_scale_ - adj, adj, nullptr, J.rec,
+                                             J.iuwt_work, nullptr, nullptr}});
+  } else if (phase_ == UPDATE || phase_ == REVERT) {
+    ops.conv.push_back({{g_.n, g_.p_full}, FftConvJob{J.model_local, J.spec_full, J.conv_full, J.fft_work, nullptr}});
+  }
+}
+
+void Task::enqueue_post(hipStream_t st) {
+  const MoresaneParams& prm = spec.prm;
+  Job& J = *job_;
+  const int sub = g_.sub;
+  const size_t s2 = (size_t)sub * sub;
+  const size_t N2 = (size_t)g_.n * g_.n;
+  switch (phase_) {
+    case DECOMP:
+      // MAD thresholds, thresholding with the edge mask and normalised maxima (main.py:278-300)
+      if (!spec.mask) mad_thresholds(J.D, g_.S, sub, sub, nm_, scal_ + kThr, J.mad_work, nullptr, st);
+      apply_threshold(J.D, J.D, g_.S, sub, sub, scal_ + kThr, prm.sigma_level,
+                      borders_.empty() ? nullptr : iflag_ + kBorders, J.pmax, nullptr, st);
+      max_final(J.pmax, threshold_grid(s2), g_.S, nullptr, scal_ + kEnergy, scal_ + kNsm, nullptr, st);
+      break;
+    case MINOR:
+      if (minor_first_) {
+        // CG set-up (main.py:365-377)
+        const int ns = max_scale_ - scale_adjust_;
+        const float* Es = J.E + (size_t)scale_adjust_ * s2;
+        fill_f32(x_, s2, 0.f, st);
+        copy_plane(J.rec, r_, s2, nullptr, st);
+        copy_plane(J.rec, J.p, s2, nullptr, st);
+        dot_partials(r_, nullptr, s2, J.partial, nullptr, st);
+        sum_partials(J.partial, reduce_grid(s2), 1, rr_, nullptr, st);
+        plane_sumsq_partials(Es, 1, (size_t)ns * s2, J.partial, nullptr, st);
+        sum_partials(J.partial, reduce_grid((size_t)ns * s2), 1, scal_ + kE2, nullptr, st);
+      }
+      break;
+    case UPDATE:
+    case REVERT:
+      // residual = dirty - H model, std of residual[sub], max of residual[sub] (main.py:497-506)
+      hipLaunchKernelGGL(k_sub, dim3(grid_for(N2)), dim3(kT), 0, st, dirty_dev(), (const float*)J.conv_full, J.resid,
+                         N2);
+      PMS_LAUNCH_CHECK();
+      window_copy(J.resid, g_.n, g_.lo, g_.lo, sub, nullptr, J.dirty_sub, st);
+      mean_std(J.dirty_sub, s2, J.partial, scal_ + kMeanSum, scal_ + kDev2, st);
+      plane_max(J.dirty_sub, 1, s2, 0, J.pmax, nullptr, st);
+      max_final(J.pmax, threshold_grid(s2), 1, (float*)(scal_ + kDmax), nullptr, nullptr, nullptr, st);
+      break;
+    default: break;
   }
 }
 
@@ -547,67 +635,8 @@ void Task::enqueue_setup(hipStream_t st) {
   max_final(J.pmax, threshold_grid(s2), 1, (float*)(scal_ + kDmax), nullptr, nullptr, nullptr, st);
 }
 
-// decomposition, MAD thresholds, (mask), thresholding and normalised maxima (main.py:278-300)
-void Task::enqueue_decomp(hipStream_t st) {
-  const MoresaneParams& prm = spec.prm;
-  Job& J = *job_;
-  const int S = g_.S;
-  const int sub = g_.sub;
-  const size_t s2 = (size_t)sub * sub;
-  iuwt_decompose(J.dirty_sub, sub, sub, S, 0, J.D, nullptr, J.iuwt_work, nullptr, nullptr, nullptr, nullptr, st);
-  mad_thresholds(J.D, S, sub, sub, nm_, scal_ + kThr, J.mad_work, nullptr, st);
-  if (spec.mask) {
-    // decomposition of dirty_sub * mask[sub] replaces the thresholded one (main.py:283-285)
-    window_copy(spec.mask, g_.n, g_.lo, g_.lo, sub, nullptr, J.cbuf, st);
-    window_copy(J.dirty_sub, sub, 0, 0, sub, J.cbuf, J.rec, st);
-    iuwt_decompose(J.rec, sub, sub, S, 0, J.D, nullptr, J.iuwt_work, nullptr, nullptr, nullptr, nullptr, st);
-  }
-  apply_threshold(J.D, J.D, S, sub, sub, scal_ + kThr, prm.sigma_level,
-                  borders_.empty() ? nullptr : iflag_ + kBorders, J.pmax, nullptr, st);
-  max_final(J.pmax, threshold_grid(s2), S, nullptr, scal_ + kEnergy, scal_ + kNsm, nullptr, st);
-}
 
-// source extraction, recomposition and CG set-up (main.py:347-377)
-void Task::enqueue_minor_start(hipStream_t st) {
-  const MoresaneParams& prm = spec.prm;
-  Job& J = *job_;
-  const int sub = g_.sub;
-  const size_t s2 = (size_t)sub * sub;
-  const int adj = scale_adjust_;
-  const int ns = max_scale_ - adj;
-  const float* tslice = J.D + (size_t)adj * s2;
-  // masks/extracted live at planes [adj, max_scale) so the decomposition can index them by scale
-  uint8_t* Ms = J.M + (size_t)adj * s2;
-  float* Es = J.E + (size_t)adj * s2;
-  source_extract(tslice, ns, sub, sub, prm.tolerance, prm.neg_comp, Ms, Es, nullptr, J.ext_work, nullptr, st);
-  iuwt_recompose(Es, ns, sub, sub, adj, nullptr, J.rec, J.iuwt_work, nullptr, nullptr, st);
-  fill_f32(x_, s2, 0.f, st);
-  copy_plane(J.rec, r_, s2, nullptr, st);
-  copy_plane(J.rec, J.p, s2, nullptr, st);
-  dot_partials(r_, nullptr, s2, J.partial, nullptr, st);
-  sum_partials(J.partial, reduce_grid(s2), 1, rr_, nullptr, st);
-  plane_sumsq_partials(Es, 1, (size_t)ns * s2, J.partial, nullptr, st);
-  sum_partials(J.partial, reduce_grid((size_t)ns * s2), 1, scal_ + kE2, nullptr, st);
-}
 
-// model += gain x, residual = dirty - H model, std of residual[sub], max of residual[sub]
-// (main.py:479-506; a negative gain reverts the update)
-void Task::enqueue_update(hipStream_t st, double gain) {
-  Job& J = *job_;
-  const size_t N2 = (size_t)g_.n * g_.n;
-  const size_t s2 = (size_t)g_.sub * g_.sub;
-  hipLaunchKernelGGL(k_model_add, dim3(grid_for(s2)), dim3(kT), 0, st, J.model_local, g_.n, g_.lo, g_.lo, g_.sub,
-                     (const float*)x_, gain);
-  PMS_LAUNCH_CHECK();
-  fft_conv(J.model_local, g_.n, g_.p_full, J.spec_full, J.conv_full, J.fft_work, nullptr, st);
-  hipLaunchKernelGGL(k_sub, dim3(grid_for(N2)), dim3(kT), 0, st, dirty_dev(), (const float*)J.conv_full, J.resid,
-                     N2);
-  PMS_LAUNCH_CHECK();
-  window_copy(J.resid, g_.n, g_.lo, g_.lo, g_.sub, nullptr, J.dirty_sub, st);
-  mean_std(J.dirty_sub, s2, J.partial, scal_ + kMeanSum, scal_ + kDev2, st);
-  plane_max(J.dirty_sub, 1, s2, 0, J.pmax, nullptr, st);
-  max_final(J.pmax, threshold_grid(s2), 1, (float*)(scal_ + kDmax), nullptr, nullptr, nullptr, st);
-}
 
 void Task::enqueue_finish(hipStream_t st) {
   Job& J = *job_;
@@ -912,13 +941,23 @@ void Runner::step() {
   for (int k : active) {
     Task& t = *tasks_[k];
     try {
-      t.enqueue(st_);
+      t.enqueue_pre(st_);
     } catch (const Error& e) {
       t.fail(e.code, e.msg);
     }
     moved = moved || t.geometry_moved();
   }
   if (moved) drop_graphs();
+  run_shared(active);
+  for (int k : active) {
+    Task& t = *tasks_[k];
+    if (t.done()) continue;
+    try {
+      t.enqueue_post(st_);
+    } catch (const Error& e) {
+      t.fail(e.code, e.msg);
+    }
+  }
   std::vector<Task*> its;
   for (int k : active)
     if (tasks_[k]->wants_iteration()) its.push_back(tasks_[k].get());
@@ -936,6 +975,23 @@ void Runner::step() {
       t.fail(e.code, e.msg);
     }
   }
+}
+
+// The shared part of every task's step, one batched call per plane side / frame.
+void Runner::run_shared(const std::vector<int>& active) {
+  SharedOps ops;
+  for (int k : active)
+    if (!tasks_[k]->done()) tasks_[k]->add_shared(ops);
+  std::map<int, std::vector<IuwtDecompJob>> dec;
+  std::map<int, std::vector<IuwtRecompJob>> rec;
+  std::map<std::pair<int, int>, std::vector<FftConvJob>> conv;
+  for (auto& e : ops.decomp) dec[e.first].push_back(e.second);
+  for (auto& e : ops.recomp) rec[e.first].push_back(e.second);
+  for (auto& e : ops.conv) conv[e.first].push_back(e.second);
+  for (auto& kv : dec) iuwt_decompose_batch(kv.second.data(), (int)kv.second.size(), kv.first, kv.first, st_);
+  for (auto& kv : rec) iuwt_recompose_batch(kv.second.data(), (int)kv.second.size(), kv.first, kv.first, st_);
+  for (auto& kv : conv)
+    fft_conv_batch(kv.second.data(), (int)kv.second.size(), kv.first.first, kv.first.second, st_);
 }
 
 // Iterations of tasks that share a plane shape and FFT frame run together, kMaxBatch per launch.
