@@ -68,7 +68,7 @@ def pct(vals, q):
     return float(np.percentile(np.asarray(vals), q)) if len(vals) else None
 
 
-def bench_batched(max_batch=32, workers=4):
+def bench_batched(max_batch=32, workers=8):
     """96 ragged pairs (32x256^2, 32x512^2, 24x1024^2, 8x2048^2), S=5: plain (one job at a time) vs
     batched (each size class stepped in lockstep, up to max_batch jobs per batched launch, classes
     concurrent on `workers` streams)."""

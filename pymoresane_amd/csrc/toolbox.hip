@@ -290,31 +290,97 @@ __device__ __forceinline__ void ccl_union(int* lab, int a, int b) {
   }
 }
 
-__global__ __launch_bounds__(kT) void k_ccl_init(const float* __restrict__ w, int* __restrict__ lab, size_t plane,
-                                                 const int* pred) {
-  if (pred_off(pred)) return;
-  const int s = blockIdx.y;
-  const float* x = w + (size_t)s * plane;
-  int* l = lab + (size_t)s * plane;
-  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < plane; i += (size_t)gridDim.x * kT)
-    l[i] = (x[i] != 0.f) ? (int)i : -1;
+// Two-level labelling: each 32x32 tile is labelled in LDS (union-find with LDS atomicMin links, so
+// every tile component's root is its smallest raster index), the tile roots are written as global
+// raster indices, then only pixels on tile borders union across tiles in global memory.  The final
+// forest, and so the roots, are those of a global union-find: every component's root is its
+// smallest raster index.
+constexpr int kCclTile = 32;
+
+__device__ __forceinline__ int lds_find(const int* L, int x) {
+  int p = L[x];
+  while (p != x) {
+    x = p;
+    p = L[x];
+  }
+  return x;
 }
 
-__global__ __launch_bounds__(kT) void k_ccl_merge(int* __restrict__ lab, int h, int wd, const int* pred) {
+__device__ __forceinline__ void lds_union(int* L, int a, int b) {
+  while (true) {
+    a = lds_find(L, a);
+    b = lds_find(L, b);
+    if (a == b) return;
+    if (a < b) {
+      const int t = a;
+      a = b;
+      b = t;
+    }
+    const int old = atomicMin(&L[a], b);
+    if (old == a) return;
+    a = old;
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_ccl_tile(const float* __restrict__ w, int* __restrict__ lab, int h, int wd,
+                                                 const int* pred) {
   if (pred_off(pred)) return;
+  constexpr int T = kCclTile, TT = kCclTile * kCclTile;
+  __shared__ int L[TT];
+  const int s = blockIdx.y;
+  const int tiles_x = (wd + T - 1) / T;
+  const int r0 = (blockIdx.x / tiles_x) * T, c0 = (blockIdx.x % tiles_x) * T;
+  const size_t plane = (size_t)h * wd;
+  const float* x = w + (size_t)s * plane;
+  int* l = lab + (size_t)s * plane;
+  for (int i = threadIdx.x; i < TT; i += kT) {
+    const int r = r0 + i / T, c = c0 + i % T;
+    L[i] = (r < h && c < wd && x[(size_t)r * wd + c] != 0.f) ? i : -1;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < TT; i += kT) {
+    if (L[i] < 0) continue;
+    const int tr = i / T, tc = i % T;
+    if (tc > 0 && L[i - 1] >= 0) lds_union(L, i, i - 1);
+    if (tr > 0) {
+      const int up = i - T;
+      if (tc > 0 && L[up - 1] >= 0) lds_union(L, i, up - 1);
+      if (L[up] >= 0) lds_union(L, i, up);
+      if (tc + 1 < T && L[up + 1] >= 0) lds_union(L, i, up + 1);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < TT; i += kT) {
+    const int r = r0 + i / T, c = c0 + i % T;
+    if (r >= h || c >= wd) continue;
+    int v = L[i];
+    if (v >= 0) {
+      v = lds_find(L, i);
+      v = (r0 + v / T) * wd + (c0 + v % T);
+    }
+    l[(size_t)r * wd + c] = v;
+  }
+}
+
+// unions across tile borders (first row, first and last column of each tile)
+__global__ __launch_bounds__(kT) void k_ccl_border(int* __restrict__ lab, int h, int wd, const int* pred) {
+  if (pred_off(pred)) return;
+  constexpr int T = kCclTile;
   const int s = blockIdx.y;
   const size_t plane = (size_t)h * wd;
   int* l = lab + (size_t)s * plane;
   for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < plane; i += (size_t)gridDim.x * kT) {
-    if (l[i] < 0) continue;
     const int r = (int)(i / wd), c = (int)(i % wd);
+    const int cr = c % T;
+    const bool top = r > 0 && r % T == 0, left = c > 0 && cr == 0, right = cr == T - 1;
+    if (!(top || left || right) || l[i] < 0) continue;
     const int me = (int)i;
-    if (c > 0 && l[i - 1] >= 0) ccl_union(l, me, me - 1);
+    if (left && l[i - 1] >= 0) ccl_union(l, me, me - 1);
     if (r > 0) {
       const int up = me - wd;
-      if (c > 0 && l[up - 1] >= 0) ccl_union(l, me, up - 1);
-      if (l[up] >= 0) ccl_union(l, me, up);
-      if (c + 1 < wd && l[up + 1] >= 0) ccl_union(l, me, up + 1);
+      if (c > 0 && (top || left) && l[up - 1] >= 0) ccl_union(l, me, up - 1);
+      if (top && l[up] >= 0) ccl_union(l, me, up);
+      if (c + 1 < wd && (top || right) && l[up + 1] >= 0) ccl_union(l, me, up + 1);
     }
   }
 }
@@ -679,9 +745,10 @@ void source_extract(const float* w, int S, int h, int wd, double tol, int neg_co
   plane_max(w, S, plane, neg_comp, pmax, pred, st);
   max_final(pmax, gx, S, smax, nullptr, nullptr, pred, st);
   const int gl = std::min(ew_grid(plane), 1024);
-  hipLaunchKernelGGL(k_ccl_init, dim3(gl, S), dim3(kT), 0, st, w, lab, plane, pred);
+  const int tiles = ceil_div(h, kCclTile) * ceil_div(wd, kCclTile);
+  hipLaunchKernelGGL(k_ccl_tile, dim3(tiles, S), dim3(kT), 0, st, w, lab, h, wd, pred);
   PMS_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_ccl_merge, dim3(gl, S), dim3(kT), 0, st, lab, h, wd, pred);
+  hipLaunchKernelGGL(k_ccl_border, dim3(gl, S), dim3(kT), 0, st, lab, h, wd, pred);
   PMS_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_ccl_compress, dim3(gl, S), dim3(kT), 0, st, lab, plane, pred);
   PMS_LAUNCH_CHECK();

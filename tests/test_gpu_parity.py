@@ -179,7 +179,8 @@ def sparse_planes(seed, s, h, w, density=0.08):
 
 @pytest.mark.parametrize("seed,shape,neg", [(0, (3, 64, 64), False), (1, (4, 128, 96), False),
                                             (2, (2, 64, 64), True), (3, (1, 33, 47), False),
-                                            (4, (5, 256, 256), False)])
+                                            (4, (5, 256, 256), False), (5, (2, 100, 70), True),
+                                            (6, (3, 600, 530), False)])
 def test_source_extraction_bit_exact(hip, seed, shape, neg):
     say("source_extraction seed=%d %s neg=%s" % (seed, shape, neg))
     w = sparse_planes(seed, *shape)
@@ -188,6 +189,18 @@ def test_source_extraction_bit_exact(hip, seed, shape, neg):
     np.testing.assert_array_equal(labels, rlabels)
     np.testing.assert_array_equal(mask, rmask)
     np.testing.assert_array_equal(ext, rext.astype(np.float32))
+
+
+def test_source_extraction_components_spanning_tiles(hip):
+    """Dense planes: components percolate across many 32x32 labelling tiles."""
+    say("source_extraction dense planes")
+    for seed, dens in ((7, 0.3), (8, 0.45)):
+        w = sparse_planes(seed, 2, 300, 260, density=dens)
+        ext, mask, labels = tools.source_extraction(w, 0.5, return_labels=True)
+        rext, rmask, rlabels = otools.source_extraction(w, 0.5, return_labels=True)
+        np.testing.assert_array_equal(labels, rlabels)
+        np.testing.assert_array_equal(mask, rmask)
+        np.testing.assert_array_equal(ext, rext.astype(np.float32))
 
 
 def test_source_extraction_empty_and_full(hip):
