@@ -350,7 +350,7 @@ struct IuwtTuning {
       : qmax(env_int("PMS_IUWT_QMAX", 16)),
         krows(env_int("PMS_IUWT_KROWS", 0)),
         target_blocks(env_int("PMS_IUWT_BLOCKS", 2048)),
-        small_pixels(env_int("PMS_IUWT_SMALL", 512 * 512)) {}
+        small_pixels(env_int("PMS_IUWT_SMALL", 256 * 256)) {}
 };
 const IuwtTuning& tuning() {
   static const IuwtTuning t;

@@ -253,15 +253,18 @@ __global__ __launch_bounds__(kT) void k_plane_max(const float* __restrict__ in, 
 }
 
 // per-plane reduce of partial maxima; out_max[s] (float) and optionally nsm[s] = max / energy[s]
-__global__ void k_max_final(const float* pmax, int nb, int S, float* out_max, const double* energy, double* nsm,
-                            const int* pred) {
+// (one block per plane; max is exact in any order)
+__global__ __launch_bounds__(kT) void k_max_final(const float* pmax, int nb, float* out_max, const double* energy,
+                                                  double* nsm, const int* pred) {
   if (pred_off(pred)) return;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= S) return;
+  const int s = blockIdx.x;
   float m = -INFINITY;
-  for (int b = 0; b < nb; ++b) m = fmaxf(m, pmax[(size_t)s * nb + b]);
-  if (out_max) out_max[s] = m;
-  if (nsm) nsm[s] = (double)m / energy[s];
+  for (int b = threadIdx.x; b < nb; b += kT) m = fmaxf(m, pmax[(size_t)s * nb + b]);
+  m = block_max_f32(m);
+  if (threadIdx.x == 0) {
+    if (out_max) out_max[s] = m;
+    if (nsm) nsm[s] = (double)m / energy[s];
+  }
 }
 
 // ----------------------------------------------------------------------------- connected components
@@ -707,7 +710,7 @@ void plane_max(const float* in, int S, size_t plane, int use_abs, float* pmax, c
 
 void max_final(const float* pmax, int nb, int S, float* out_max, const double* energy, double* nsm, const int* pred,
                hipStream_t st) {
-  hipLaunchKernelGGL(k_max_final, dim3(ceil_div(S, 64)), dim3(64), 0, st, pmax, nb, S, out_max, energy, nsm, pred);
+  hipLaunchKernelGGL(k_max_final, dim3(S), dim3(kT), 0, st, pmax, nb, out_max, energy, nsm, pred);
   PMS_LAUNCH_CHECK();
 }
 
