@@ -358,7 +358,13 @@ __global__ __launch_bounds__(kThreads) void k_cols_fwd(const float2* __restrict_
 // padding, and only half of every inverse output is kept.  Storage order, twiddles and arithmetic
 // order per butterfly are those of the generic kernels above (same digit-reversed layout).
 
-constexpr int padc(int i) { return i + (i >> 5); }
+// LDS slot of element i: one pad per 32 and one more per 256 elements.  The second term spreads the
+// digit-reversed (stride-256) accesses of the real-FFT split over all banks; both terms are linear
+// over every butterfly's elements (base + k*M never carries into them), so a butterfly's offsets
+// stay compile-time constants.  span() is a transform's LDS length, odd so that the transforms of
+// one block start on different banks.
+__host__ __device__ constexpr int padc(int i) { return i + (i >> 5) + (i >> 8); }
+__host__ __device__ constexpr int span(int n) { return padc(n) + 1; }
 
 template <int LG>
 struct Plan {
@@ -424,7 +430,7 @@ __device__ __forceinline__ void stage16(float2* sm, int lt, const StageTw<LG, NT
     const int b = lt + j * NT;
     if ((Pl::NBF % NT) != 0 && b >= Pl::NBF) break;
     const int base = ((b >> LM) << (LM + 4)) + (b & (M - 1));
-    float2* s0 = sm + (base + (base >> 5));
+    float2* s0 = sm + padc(base);
     float2 x[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) x[k] = s0[padc(k * M)];
@@ -483,7 +489,7 @@ struct Stage0 {
 #pragma unroll
         for (int k = 1; k < Pl::R0; ++k) x[u][k] = cmul(x[u][k], wk[k]);
       }
-      float2* s0 = sm + (n + (n >> 5));
+      float2* s0 = sm + padc(n);
 #pragma unroll
       for (int k = 0; k < Pl::R0; ++k) s0[padc(k * Pl::M0)] = x[u][k];
     }
@@ -495,7 +501,7 @@ struct Stage0 {
     for (int u = 0; u < U; ++u) {
       const int n = lt + u * NT;
       if ((Pl::M0 % NT) != 0 && n >= Pl::M0) break;
-      const float2* s0 = sm + (n + (n >> 5));
+      const float2* s0 = sm + padc(n);
       float2 y[Pl::R0];
 #pragma unroll
       for (int k = 0; k < Pl::R0; ++k) y[k] = s0[padc(k * Pl::M0)];
@@ -522,14 +528,14 @@ struct RowCfg {
   static constexpr int NT = (L / 32 > 64) ? L / 32 : 64;
   static constexpr int rb() {
     int r = 16;
-    while (r > 1 && (r * NT > 512 || (size_t)r * (L + (L >> 5)) * sizeof(float2) > 140 * 1024 || N / r < 256))
+    while (r > 1 && (r * NT > 512 || (size_t)r * span(L) * sizeof(float2) > 140 * 1024 || N / r < 256))
       r >>= 1;
     return r;
   }
   static constexpr int RB = rb();
   static constexpr int T = RB * NT;
   static constexpr int NPP = (RB * (L / 2 + 1) + T - 1) / T;  // real-FFT pairs per thread
-  static constexpr size_t LDS = (size_t)RB * (L + (L >> 5)) * sizeof(float2);
+  static constexpr size_t LDS = (size_t)RB * span(L) * sizeof(float2);
 };
 
 template <int PG>
@@ -545,7 +551,7 @@ struct ColCfg {
   static constexpr int CPB = cpb();
   static constexpr int NT = (P / (256 / CPB) > 32) ? P / 32 : 256 / CPB;
   static constexpr int T = CPB * NT;
-  static constexpr size_t LDS = (size_t)CPB * (P + (P >> 5)) * sizeof(float2);
+  static constexpr size_t LDS = (size_t)CPB * span(P) * sizeof(float2);
 };
 
 // storage position of frequency k (compile-time plan of length 2^LG)
@@ -569,7 +575,7 @@ template <int PG, bool LIN>
 __global__ __launch_bounds__((RowCfg<PG, LIN>::T)) void k_conv_r2c(R2cBatch bt, int n_rows, int ldT, const float2* __restrict__ tw) {
   using C = RowCfg<PG, LIN>;
   constexpr int LG = PG - 1;
-  constexpr int L = C::L, RB = C::RB, NT = C::NT, SP = L + (L >> 5);
+  constexpr int L = C::L, RB = C::RB, NT = C::NT, SP = span(L);
   const int jb = blockIdx.y;
   const int* pred = bt.pred[jb];
   if (pred && *pred == 0) return;
@@ -613,8 +619,8 @@ __global__ __launch_bounds__((RowCfg<PG, LIN>::T)) void k_conv_r2c(R2cBatch bt, 
     if (k > L / 2 || r2 >= n_rows) continue;
     const float2* sm2 = smem + g2 * SP;
     const int pk = pos_of<LG>(k & (L - 1)), pl = pos_of<LG>((L - k) & (L - 1));
-    const float2 zk = sm2[pk + (pk >> 5)];
-    const float2 zl = sm2[pl + (pl >> 5)];
+    const float2 zk = sm2[padc(pk)];
+    const float2 zl = sm2[padc(pl)];
     const float2 w = wpp[i];
     {
       const float2 c = cconj(zl);
@@ -637,7 +643,7 @@ template <int PG, bool LIN>
 __global__ __launch_bounds__(ColCfg<PG>::T) void k_conv_cols(ColBatch bt, int n_rows, const float2* __restrict__ tw) {
   using C = ColCfg<PG>;
   using Pl = Plan<PG>;
-  constexpr int P = C::P, NT = C::NT, SP = P + (P >> 5);
+  constexpr int P = C::P, NT = C::NT, SP = span(P);
   constexpr int NCOLS = P / 2 + 1;
   const int jb = blockIdx.y;
   const int* pred = bt.pred[jb];
@@ -688,7 +694,7 @@ __global__ __launch_bounds__(ColCfg<PG>::T) void k_conv_cols(ColBatch bt, int n_
       const int b = lt + j * NT;
       if ((Pl::NBF % NT) != 0 && b >= Pl::NBF) break;
       const int base = 16 * b;
-      float2* s1 = sm + (base + (base >> 5));
+      float2* s1 = sm + padc(base);
       float2 x[16];
 #pragma unroll
       for (int m = 0; m < 16; ++m) x[m] = s1[m];
@@ -715,7 +721,7 @@ __global__ __launch_bounds__((RowCfg<PG, LIN>::T)) void k_conv_c2r(C2rBatch bt, 
                                                   const float2* __restrict__ tw) {
   using C = RowCfg<PG, LIN>;
   constexpr int LG = PG - 1;
-  constexpr int L = C::L, RB = C::RB, NT = C::NT, SP = L + (L >> 5);
+  constexpr int L = C::L, RB = C::RB, NT = C::NT, SP = span(L);
   const int jb = blockIdx.y;
   const int* pred = bt.pred[jb];
   if (pred && *pred == 0) return;
@@ -764,14 +770,14 @@ __global__ __launch_bounds__((RowCfg<PG, LIN>::T)) void k_conv_c2r(C2rBatch bt, 
       const float2 e = cadd(xk[i], cconj(xl[i]));
       const float2 o = cmul(csub(xk[i], cconj(xl[i])), cconj(w));
       const int pk = pos_of<LG>(k & (L - 1));
-      sm2[pk + (pk >> 5)] = make_float2(e.x - o.y, e.y + o.x);
+      sm2[padc(pk)] = make_float2(e.x - o.y, e.y + o.x);
     }
     const int kl = L - k;
     if (kl != k && kl < L) {
       const float2 e = cadd(xl[i], cconj(xk[i]));
       const float2 o = cmul(csub(xl[i], cconj(xk[i])), make_float2(-w.x, -w.y));  // conj(w^(L-k)) = -w^k
       const int pl = pos_of<LG>(kl);
-      sm2[pl + (pl >> 5)] = make_float2(e.x - o.y, e.y + o.x);
+      sm2[padc(pl)] = make_float2(e.x - o.y, e.y + o.x);
     }
   }
   __syncthreads();
