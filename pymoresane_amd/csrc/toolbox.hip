@@ -29,10 +29,43 @@ constexpr int kT = 256;
 __device__ __forceinline__ bool pred_off(const int* pred) { return pred && *pred == 0; }
 
 // ----------------------------------------------------------------------------- copies
-__global__ void k_copy(const float* __restrict__ src, float* __restrict__ dst, size_t n, const int* pred) {
+// Streaming copy/fill: 16-byte accesses, four per thread in flight, for the 16-byte-aligned bulk
+// (the host checks alignment); the last n % 4 elements elementwise.
+__global__ __launch_bounds__(kT) void k_copy(const float* __restrict__ src, float* __restrict__ dst, size_t n,
+                                             const int* pred) {
+  if (pred_off(pred)) return;
+  const size_t n4 = n >> 2;
+  const float4* __restrict__ s4 = reinterpret_cast<const float4*>(src);
+  float4* __restrict__ d4 = reinterpret_cast<float4*>(dst);
+  const size_t stride = (size_t)gridDim.x * kT;
+  size_t i = blockIdx.x * (size_t)kT + threadIdx.x;
+  for (; i + 3 * stride < n4; i += 4 * stride) {
+    const float4 a = s4[i], b = s4[i + stride], c = s4[i + 2 * stride], d = s4[i + 3 * stride];
+    d4[i] = a;
+    d4[i + stride] = b;
+    d4[i + 2 * stride] = c;
+    d4[i + 3 * stride] = d;
+  }
+  for (; i < n4; i += stride) d4[i] = s4[i];
+  for (size_t t = (n4 << 2) + blockIdx.x * (size_t)kT + threadIdx.x; t < n; t += stride) dst[t] = src[t];
+}
+
+__global__ void k_copy1(const float* __restrict__ src, float* __restrict__ dst, size_t n, const int* pred) {
   if (pred_off(pred)) return;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     dst[i] = src[i];
+}
+
+// n elements of a 4-byte type, p 16-byte aligned
+template <class T>
+__global__ __launch_bounds__(kT) void k_fill16(T* __restrict__ p, size_t n, T v) {
+  const size_t n4 = n >> 2;
+  uint4 w;
+  w.x = w.y = w.z = w.w = *reinterpret_cast<const uint32_t*>(&v);
+  uint4* __restrict__ p4 = reinterpret_cast<uint4*>(p);
+  const size_t stride = (size_t)gridDim.x * kT;
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n4; i += stride) p4[i] = w;
+  for (size_t t = (n4 << 2) + blockIdx.x * (size_t)kT + threadIdx.x; t < n; t += stride) p[t] = v;
 }
 
 __global__ void k_masked_copy(const float* __restrict__ src, const uint8_t* __restrict__ m, float* __restrict__ dst,
@@ -582,8 +615,23 @@ static int ew_grid(size_t n) {
 
 int reduce_grid(size_t n) { return ew_grid(n); }
 
+static bool aligned16(const void* a, const void* b = nullptr) {
+  return ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & 15) == 0;
+}
+
+// blocks for a 16-byte streaming kernel doing four accesses per thread
+static int vec_grid(size_t n) {
+  size_t b = ((n >> 2) + (size_t)kT * 4 - 1) / ((size_t)kT * 4);
+  if (b < 1) b = 1;
+  if (b > 16384) b = 16384;
+  return (int)b;
+}
+
 void copy_plane(const float* src, float* dst, size_t n, const int* pred, hipStream_t st) {
-  hipLaunchKernelGGL(k_copy, dim3(ew_grid(n)), dim3(kT), 0, st, src, dst, n, pred);
+  if (aligned16(src, dst))
+    hipLaunchKernelGGL(k_copy, dim3(vec_grid(n)), dim3(kT), 0, st, src, dst, n, pred);
+  else
+    hipLaunchKernelGGL(k_copy1, dim3(ew_grid(n)), dim3(kT), 0, st, src, dst, n, pred);
   PMS_LAUNCH_CHECK();
 }
 
@@ -593,12 +641,18 @@ void masked_copy(const float* src, const uint8_t* mask, float* dst, size_t n, co
 }
 
 void fill_f32(float* p, size_t n, float v, hipStream_t st) {
-  hipLaunchKernelGGL(k_fill_f32, dim3(ew_grid(n)), dim3(kT), 0, st, p, n, v);
+  if (aligned16(p))
+    hipLaunchKernelGGL(k_fill16<float>, dim3(vec_grid(n)), dim3(kT), 0, st, p, n, v);
+  else
+    hipLaunchKernelGGL(k_fill_f32, dim3(ew_grid(n)), dim3(kT), 0, st, p, n, v);
   PMS_LAUNCH_CHECK();
 }
 
 void fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t st) {
-  hipLaunchKernelGGL(k_fill_u32, dim3(ew_grid(n)), dim3(kT), 0, st, p, n, v);
+  if (aligned16(p))
+    hipLaunchKernelGGL(k_fill16<uint32_t>, dim3(vec_grid(n)), dim3(kT), 0, st, p, n, v);
+  else
+    hipLaunchKernelGGL(k_fill_u32, dim3(ew_grid(n)), dim3(kT), 0, st, p, n, v);
   PMS_LAUNCH_CHECK();
 }
 
