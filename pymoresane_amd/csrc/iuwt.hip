@@ -151,7 +151,7 @@ enum WalkFlags { F_OUTC = 1, F_OUTW = 2, F_MASK = 4, F_REF = 8 };
 // Output slots of steps that produce no pixel (the 2H lead-in rows, padding steps, dead lanes):
 // their stores land here instead of being branched around, which keeps every step's sequence
 // of memory operations identical.
-__device__ float g_walk_sink[kThreads];
+__device__ float g_walk_sink[1024];
 
 // Row walk with group pipelining over the comb/phase decomposition described at the top of this
 // file; the walk is cut into groups of K rows: while a group's K steps run, the input
@@ -214,8 +214,11 @@ __global__ __launch_bounds__(kThreads) void k_iuwt_walk(ScaleParams sp) {
     const int qq = e & (Q - 1);
     ld_col[m] = ld_ok[m] ? fold_idx(cg * Q + qq + (tb * TJ + jj - H) * d, w) : 0;
   }
+  // input row of step s; the prefetch of the two groups past the walk's end re-reads the last
+  // row, so every row index stays within the extension's range (at d = h/2 the padding steps
+  // would otherwise reach beyond two periods)
   auto row_of = [&](int s) {
-    return fold_idx(p + (i_begin + s) * d, h);
+    return fold_idx(p + (i_begin + min(s, n_steps - 1)) * d, h);
   };
   // output pixel of step s (clamped to a real pixel when the step has none)
   auto out_off = [&](int s) {
@@ -333,8 +336,252 @@ __global__ __launch_bounds__(kThreads) void k_iuwt_walk(ScaleParams sp) {
   }
 }
 
+// ---- two scales per launch ----------------------------------------------------------------
+// Operands of one job of a two-scale recomposition: R_s = h_s (h_{s+1} R + w_{s+1}) + w_s.  In
+// the frame of dilation d = 2^s the coarse scale is an undilated filter of the frame subsampled by
+// 2, so one walk serves both: the first stage filters at tooth/row spacing 2, the second at
+// spacing 1.
+struct PairJob {
+  const float* in;       // R_{s+2} (or the coarsest detail plane)
+  float* out;            // R_s
+  const float* w_in0;    // detail plane of scale s
+  const float* w_in1;    // detail plane of scale s+1
+  const uint8_t* mask0;  // masks of those planes (null = none)
+  const uint8_t* mask1;
+  const int* pred;
+};
+struct PairParams {
+  PairJob job[kMaxBatch];
+  int h, w, d, q_log2, tj, k_rows, n_cg, n_tb, n_seg;
+};
+
+// Whether the symmetric extension maps index i (in [-2n, 4n)) through a mirror.
+__device__ __forceinline__ bool fold_mirrors(int i, int n) {
+  i = i < 0 ? i + 2 * n : i;
+  i = i >= 2 * n ? i - 2 * n : i;
+  return i >= n;
+}
+
+// Raw buffer access to one plane: loads take a per-lane byte offset (VGPR, fixed for the walk)
+// and a per-row byte offset (SGPR), so a row's loads need no per-lane address arithmetic.  Plane
+// sizes stay below 2^31 bytes (checked on the host); every offset used is inside the plane.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float buf_f32(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ int buf_u8(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  return (int)__builtin_amdgcn_raw_buffer_load_b8(r, (int)voff, (int)soff, 0);
+}
+
+// Two-scale recomposition walk.  Per input row of R (staged in LDS with a halo of 6 teeth) each
+// thread computes the intermediate T = h_{s+1} R + w_{s+1} for one T lane (teeth -2 .. TJ+1 of
+// the comb: the plan keeps (TJ+4)*Q <= NT, the block size), staged in a second LDS row, and one step later
+// R_s = h_s T + w_s for its output lane.  That lag gives one barrier per step: at step s the
+// barrier publishes R(s) and T(s-1); stage 1 reads R(s) and writes T(s), stage 2 reads T(s-1), and
+// R(s+1) is written last, all into double buffers.  A walk has at most 64 steps: the row offsets
+// of step s (input, T and output rows, folded) are computed once, one step per lane, and read
+// with readlane, and the rows whose T is mirrored form one 64-bit mask.
+//
+// The result is bit-identical to two one-scale launches.  There the fine scale reads the stored
+// T at folded positions; here T is computed at the unfolded position.  On the symmetric extension
+// T(x) = T(fold(x)) in exact arithmetic, and the rounded sums agree too when the operands are
+// accumulated in the same order: through a mirror the neighbours of x appear in reverse order, so
+// mirrored lanes (columns) and mirrored rows accumulate their 5 taps from the other end.
+constexpr int kPairMaxSteps = 64;
+
+template <int FL, int K, int ML, int NT>
+__global__ __launch_bounds__(NT) void k_iuwt_recomp2(PairParams sp) {
+  constexpr bool MASK = (FL & F_MASK) != 0;
+  constexpr int HR = 6, HT = 2;
+  constexpr int LAG_T = 4, LAG_O = 6;  // T row = input row - 4, output row = input row - 6
+  static_assert(K % 2 == 0, "group length must be even (LDS double buffer parity)");
+  const PairJob a = sp.job[blockIdx.y];
+  if (a.pred && *a.pred == 0) return;
+
+  // LDS: two R rows of ML*NT slots and two T rows of NT slots (every lane writes its slot)
+  extern __shared__ float lds[];
+  float* rbuf0 = lds;
+  float* rbuf1 = lds + ML * NT;
+  float* tbuf0 = lds + 2 * ML * NT;
+  float* tbuf1 = tbuf0 + NT;
+  const int Q = 1 << sp.q_log2;
+  const int TJ = sp.tj;
+  const int r_elems = (TJ + 2 * HR) * Q;
+  const int t_elems = (TJ + 2 * HT) * Q;
+
+  const unsigned nblk = gridDim.x;
+  unsigned b = xcd_remap(blockIdx.x, nblk);
+  const int seg = b % sp.n_seg;
+  b /= sp.n_seg;
+  const int tb = b % sp.n_tb;
+  b /= sp.n_tb;
+  const int cg = b % sp.n_cg;
+  const int p = b / sp.n_cg;
+  const int d = sp.d;
+  const int h = sp.h, w = sp.w;
+
+  const int t = threadIdx.x;
+  const int q = t & (Q - 1);
+  const int j = t >> sp.q_log2;
+  const bool thread_live = j < TJ;
+  const int col = cg * Q + q + (tb * TJ + j) * d;
+  const bool col_ok = thread_live && col < w;
+  const unsigned col_off = col_ok ? 4u * (unsigned)col : 0u;
+  const int t2_lds = thread_live ? (j + HT) * Q + q : 2 * Q;  // stage-2 taps (dead lanes: valid slots)
+
+  // this thread's T lane: tooth j - HT of the comb
+  const bool tl_ok = t < t_elems;
+  const int tx = cg * Q + q + (tb * TJ + j - HT) * d;
+  const unsigned tl_off = tl_ok ? 4u * (unsigned)fold_idx(tx, w) : 0u;
+  const bool tl_mir = tl_ok && fold_mirrors(tx, w);
+  const int tl_lds = tl_ok ? (j - HT + HR) * Q + q : 4 * Q;  // dead lanes read valid slots
+  const bool wave_mir = __any(tl_mir);
+
+  const int mr = (h - p + d - 1) / d;
+  const int o_begin = seg * sp.k_rows;
+  const int o_end = min(o_begin + sp.k_rows, mr);
+  const int n_out = o_end - o_begin;
+  const int i_begin = o_begin - LAG_O;
+  const int n_steps = n_out + 2 * LAG_O;  // <= kPairMaxSteps (plan)
+  // one more step than rows: stage 2 of the last row runs in it
+  const int n_groups = (n_steps + 1 + K - 1) / K;
+
+  // per-step row byte offsets, step = lane: input rows clamped to the walk (the prefetch past the
+  // end re-reads the last row), T rows before the first complete T window use the first real one
+  const int lane = t & 63;
+  const int ls = min(lane, n_steps - 1);
+  const int tab_in = fold_idx(p + (i_begin + ls) * d, h) * w * 4;
+  const int tab_t = fold_idx(p + (i_begin + max(ls, 2 * LAG_T) - LAG_T) * d, h) * w * 4;
+  const int tab_out = (p + min(max(i_begin + lane - LAG_O, o_begin), o_end - 1) * d) * w * 4;
+  const unsigned long long row_mir = __ballot(fold_mirrors(p + (i_begin + lane - LAG_T) * d, h));
+  auto step_row = [](int tab, int s) { return (unsigned)__builtin_amdgcn_readlane(tab, min(s, kPairMaxSteps - 1)); };
+
+  unsigned ld_off[ML];
+#pragma unroll
+  for (int m = 0; m < ML; ++m) {
+    const int e = t + m * NT;
+    const int jj = e >> sp.q_log2;
+    const int qq = e & (Q - 1);
+    ld_off[m] = e < r_elems ? 4u * (unsigned)fold_idx(cg * Q + qq + (tb * TJ + jj - HR) * d, w) : 0u;
+  }
+  const unsigned plane = (unsigned)h * (unsigned)w;
+  const auto r_in = plane_rsrc(a.in, 4u * plane);
+  const auto r_w0 = plane_rsrc(a.w_in0, 4u * plane);
+  const auto r_w1 = plane_rsrc(a.w_in1, 4u * plane);
+  const auto r_m0 = plane_rsrc(a.mask0, plane);
+  const auto r_m1 = plane_rsrc(a.mask1, plane);
+
+  struct Group {
+    float row[K][ML];
+    float w1[K];
+    int m1[K];
+    float w0[K];
+    int m0[K];
+  };
+  auto load_group = [&](Group& G, int g) {
+#pragma unroll
+    for (int u = 0; u < K; ++u) {
+      const int s = g * K + u;
+      const unsigned s_in = step_row(tab_in, s), s_t = step_row(tab_t, s), s_o = step_row(tab_out, s);
+#pragma unroll
+      for (int m = 0; m < ML; ++m) G.row[u][m] = buf_f32(r_in, ld_off[m], s_in);
+      G.w1[u] = buf_f32(r_w1, tl_off, s_t);
+      G.w0[u] = buf_f32(r_w0, col_off, s_o);
+      if constexpr (MASK) {
+        G.m1[u] = buf_u8(r_m1, tl_off >> 2, s_t >> 2);
+        G.m0[u] = buf_u8(r_m0, col_off >> 2, s_o >> 2);
+      }
+    }
+  };
+  auto put_row = [&](float* buf, const float* row) {
+#pragma unroll
+    for (int m = 0; m < ML; ++m) buf[t + m * NT] = row[m];
+  };
+
+  float vt[9];
+  float vr[5];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) vt[k] = 0.f;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) vr[k] = 0.f;
+  float w0_prev = 0.f;
+
+  auto step = [&](const Group& G, const Group& Gn, int u, int s) {
+    float* rcur = (u & 1) ? rbuf1 : rbuf0;
+    float* tcur = (u & 1) ? tbuf1 : tbuf0;
+    float* rnext = (u & 1) ? rbuf0 : rbuf1;
+    const float* tprev = (u & 1) ? tbuf0 : tbuf1;
+    __syncthreads();
+    // stage 1, scale s+1 (taps at tooth spacing 2 and row spacing 2): T of step s
+    {
+      float x[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) x[k] = rcur[tl_lds + (2 * k - 4) * Q];
+      float hr = 0.f;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) hr = fmaf(c_h5[k], x[k], hr);
+      if (wave_mir) {
+        float hm = 0.f;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) hm = fmaf(c_h5[k], x[4 - k], hm);
+        hr = tl_mir ? hm : hr;
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) vt[k] = vt[k + 1];
+      vt[8] = hr;
+      float c = 0.f;
+      if ((row_mir >> (s & 63)) & 1) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) c = fmaf(c_h5[k], vt[8 - 2 * k], c);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) c = fmaf(c_h5[k], vt[2 * k], c);
+      }
+      float wi = G.w1[u];
+      if constexpr (MASK) wi = G.m1[u] == 0 ? 0.f : wi;
+      tcur[t] = c + wi;
+    }
+    // stage 2, scale s (taps at spacing 1): output of step s-1
+    {
+      float ht = 0.f;
+#pragma unroll
+      for (int k = -2; k <= 2; ++k) ht = fmaf(c_h5[k + 2], tprev[t2_lds + k * Q], ht);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) vr[k] = vr[k + 1];
+      vr[4] = ht;
+      float c = 0.f;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) c = fmaf(c_h5[k], vr[k], c);
+      const int so = s - 1;
+      const bool out = (unsigned)(so - 2 * LAG_O) < (unsigned)n_out && col_ok;
+      float* dst = reinterpret_cast<float*>(reinterpret_cast<char*>(a.out) + step_row(tab_out, max(so, 0)) + col_off);
+      *(out ? dst : g_walk_sink + t) = c + w0_prev;
+    }
+    float wi = G.w0[u];
+    if constexpr (MASK) wi = G.m0[u] == 0 ? 0.f : wi;
+    w0_prev = wi;
+    put_row(rnext, u + 1 < K ? G.row[u + 1] : Gn.row[0]);
+  };
+
+  Group GA, GB;
+  load_group(GA, 0);
+  put_row(rbuf0, GA.row[0]);
+  for (int g = 0; g < n_groups; g += 2) {
+    load_group(GB, g + 1);
+#pragma unroll
+    for (int u = 0; u < K; ++u) step(GA, GB, u, g * K + u);
+    load_group(GA, g + 2);
+    if (g + 1 >= n_groups) break;
+#pragma unroll
+    for (int u = 0; u < K; ++u) step(GB, GA, u, (g + 1) * K + u);
+  }
+}
+
 struct Plan {
   int q_log2, tj, k_rows, n_cg, n_tb, n_seg, blocks, lds_bytes;
+  int nt = kThreads;  // threads per block
 };
 
 int env_int(const char* name, int dflt) {
@@ -346,11 +593,13 @@ int env_int(const char* name, int dflt) {
 struct IuwtTuning {
   int qmax, krows, target_blocks;
   long small_pixels;  // planes up to this many pixels use the direct-stencil kernel
+  int pair_min_rows;  // two-scale recomposition needs this many rows per row phase of the finer scale
   IuwtTuning()
       : qmax(env_int("PMS_IUWT_QMAX", 16)),
         krows(env_int("PMS_IUWT_KROWS", 0)),
         target_blocks(env_int("PMS_IUWT_BLOCKS", 2048)),
-        small_pixels(env_int("PMS_IUWT_SMALL", 256 * 256)) {}
+        small_pixels(env_int("PMS_IUWT_SMALL", 256 * 256)),
+        pair_min_rows(env_int("PMS_IUWT_PAIR_MIN_ROWS", 8)) {}
 };
 const IuwtTuning& tuning() {
   static const IuwtTuning t;
@@ -426,6 +675,104 @@ void launch_walk(ScaleParams& sp, int count, const Plan& pl, int halo, int fl, h
       launch_walk_fl<MODE, 0>(sp, count, pl, halo, st);
   } else {
     launch_walk_fl<MODE, 0>(sp, count, pl, halo, st);
+  }
+}
+
+// Plan of a two-scale recomposition at scales (s, s+1) on the frame of scale s, or false when the
+// pair is not used for this shape (small planes, too few rows per row phase).  The comb keeps the
+// T row ((TJ+4) teeth of Q phases) within one lane per thread.
+bool recomp2_plan(int h, int w, int s, Plan* out) {
+  if ((long)h * w <= tuning().small_pixels || (long)h * w >= (1L << 29)) return false;
+  if ((4L << s) > h || (4L << s) > w) return false;
+  const int d = 1 << s;
+  const int mr = ceil_div(h, d);
+  if (mr < tuning().pair_min_rows) return false;
+  const int teeth = ceil_div(w, d);
+  int q = std::min(d, tuning().qmax);
+  if (q > 32) q = 32;
+  // wide chunks take 512-thread blocks so the 4 halo teeth of the T row cost a small share
+  const int nt = q >= 16 ? 2 * kThreads : kThreads;
+  auto tj_for = [&](int qq) { return std::min(nt / qq - 4, teeth); };
+  // widen the chunk while that gives more live output lanes (few teeth)
+  while (q < d && 2 * q <= 32 && tj_for(2 * q) * 2 * q > tj_for(q) * q) q *= 2;
+  const int tj = tj_for(q);
+  if (tj < 1) return false;
+  Plan pl;
+  pl.nt = nt;
+  pl.q_log2 = ilog2(q);
+  pl.tj = tj;
+  const int strips = d * ceil_div(d, q) * ceil_div(teeth, tj);
+  int k = tuning().krows;
+  if (k <= 0) {
+    const long target = tuning().target_blocks;
+    long segs = (target + strips - 1) / strips;
+    if (segs < 1) segs = 1;
+    k = ceil_div(mr, (int)segs);
+    if (k < 24) k = 24;
+  }
+  if (k > kPairMaxSteps - 12) k = kPairMaxSteps - 12;  // a walk's steps fit one wave's lanes
+  if (k > mr) k = mr;
+  pl.k_rows = k;
+  pl.n_cg = ceil_div(d, q);
+  pl.n_tb = ceil_div(teeth, tj);
+  pl.n_seg = ceil_div(mr, k);
+  pl.blocks = d * pl.n_cg * pl.n_tb * pl.n_seg;
+  const int r_elems = (tj + 12) * q;
+  const int ml = r_elems <= 2 * nt ? 2 : (r_elems <= 4 * nt ? 4 : 8);
+  pl.lds_bytes = 2 * (ml + 1) * nt * (int)sizeof(float);
+  *out = pl;
+  return true;
+}
+
+template <int FL, int NT>
+void launch_recomp2_nt(PairParams& sp, int count, const Plan& pl, hipStream_t st) {
+  const int elems = (pl.tj + 12) << pl.q_log2;
+  dim3 grid(pl.blocks, count), block(NT);
+  if (elems <= 2 * NT)
+    hipLaunchKernelGGL((k_iuwt_recomp2<FL, kWalkGroup, 2, NT>), grid, block, pl.lds_bytes, st, sp);
+  else if (elems <= 4 * NT)
+    hipLaunchKernelGGL((k_iuwt_recomp2<FL, kWalkGroup, 4, NT>), grid, block, pl.lds_bytes, st, sp);
+  else
+    hipLaunchKernelGGL((k_iuwt_recomp2<FL, kWalkGroup, 8, NT>), grid, block, pl.lds_bytes, st, sp);
+  PMS_LAUNCH_CHECK();
+}
+
+template <int FL>
+void launch_recomp2_fl(PairParams& sp, int count, const Plan& pl, hipStream_t st) {
+  if (pl.nt == 2 * kThreads)
+    launch_recomp2_nt<FL, 2 * kThreads>(sp, count, pl, st);
+  else
+    launch_recomp2_nt<FL, kThreads>(sp, count, pl, st);
+}
+
+// jobs[0..count) share shape and scale pair; masked and unmasked jobs go to separate launches
+void recomp2_batch(const PairJob* jobs, int count, int h, int w, int s, const Plan& pl, hipStream_t st) {
+  PairParams sp{};
+  sp.h = h;
+  sp.w = w;
+  sp.d = 1 << s;
+  sp.q_log2 = pl.q_log2;
+  sp.tj = pl.tj;
+  sp.k_rows = pl.k_rows;
+  sp.n_cg = pl.n_cg;
+  sp.n_tb = pl.n_tb;
+  sp.n_seg = pl.n_seg;
+  for (int masked = 0; masked < 2; ++masked) {
+    int nb = 0;
+    auto flush = [&]() {
+      if (nb == 0) return;
+      if (masked)
+        launch_recomp2_fl<F_MASK>(sp, nb, pl, st);
+      else
+        launch_recomp2_fl<0>(sp, nb, pl, st);
+      nb = 0;
+    };
+    for (int k = 0; k < count; ++k) {
+      if ((jobs[k].mask0 != nullptr) != (masked != 0)) continue;
+      sp.job[nb++] = jobs[k];
+      if (nb == kMaxBatch) flush();
+    }
+    flush();
   }
 }
 
@@ -636,22 +983,35 @@ void iuwt_recompose(const float* detail, int nplanes, int h, int w, int scale_ad
 }
 
 // From the coarsest scale down: at scale s a job adds its detail plane (scale_adjust <= s) or only
-// smooths (s < scale_adjust); jobs of the same kind share one launch.  Without a smoothed base R
+// smooths (s < scale_adjust).  Pairs of consecutive detail scales run as one two-scale launch where
+// the shape allows it (the same planes result).  Each job's steps form a list; round r issues every
+// job's r-th step, steps of the same kind and scale sharing one launch.  Without a smoothed base R
 // starts at zero, so a job's coarsest step is its (masked) detail plane itself.
 void iuwt_recompose_batch(const IuwtRecompJob* jobs, int count, int h, int w, hipStream_t st) {
   const size_t plane = (size_t)h * w;
+  enum { OP_RECOMP, OP_SMOOTH, OP_PAIR };
+  struct Op {
+    int kind, s;  // OP_PAIR covers scales s+1 then s
+  };
   struct State {
     const float* cur;
-    int slot, s_start, remaining;
+    int slot;
+    std::vector<Op> ops;
   };
   std::vector<State> state(count);
-  int smax = -1;
+  size_t rounds = 0;
+  // PMS_IUWT_PAIR=0 keeps every scale in its own launch, 1 pairs detail scales from the coarsest
+  // down, 2 from the finest up (read per call; the results are identical)
+  const int pair = env_int("PMS_IUWT_PAIR", 2);
   for (int k = 0; k < count; ++k) {
     const IuwtRecompJob& j = jobs[k];
     PMS_REQUIRE(j.nplanes >= 1 && j.scale_adjust >= 0, "recomposition needs at least one detail plane");
     const int top = j.nplanes + j.scale_adjust;
     check_iuwt_scale(h, w, top - 1);
-    State sk{j.smoothed, 0, top - 1, 0};
+    State& sk = state[k];
+    sk.cur = j.smoothed;
+    sk.slot = 0;
+    int s = top - 1;
     if (!j.smoothed) {
       const size_t off = (size_t)(top - 1 - j.scale_adjust) * plane;
       if (!j.mask) {
@@ -661,48 +1021,83 @@ void iuwt_recompose_batch(const IuwtRecompJob* jobs, int count, int h, int w, hi
         sk.cur = j.work;
         sk.slot = 1;
       }
-      --sk.s_start;
+      --s;
     }
-    sk.remaining = sk.s_start + 1;  // launches still to issue
-    if (sk.remaining == 0) copy_plane(sk.cur, j.out, plane, j.pred, st);
-    state[k] = sk;
-    smax = std::max(smax, sk.s_start);
+    Plan pl;
+    // pairing from the finest scale up leaves the odd scale out at the top
+    bool skip_top = pair == 2 && s >= j.scale_adjust && (s - j.scale_adjust + 1) % 2 == 1;
+    while (s >= 0) {
+      if (skip_top) {
+        skip_top = false;
+        sk.ops.push_back({OP_RECOMP, s});
+        --s;
+        continue;
+      }
+      if (pair && s - 1 >= j.scale_adjust && recomp2_plan(h, w, s - 1, &pl)) {
+        sk.ops.push_back({OP_PAIR, s - 1});
+        s -= 2;
+      } else {
+        sk.ops.push_back({s >= j.scale_adjust ? OP_RECOMP : OP_SMOOTH, s});
+        --s;
+      }
+    }
+    if (sk.ops.empty()) copy_plane(sk.cur, j.out, plane, j.pred, st);
+    rounds = std::max(rounds, sk.ops.size());
   }
   std::vector<IuwtScaleCall> calls;
+  std::vector<PairJob> pairs;
   std::vector<int> who;
-  calls.reserve(count);
-  who.reserve(count);
-  for (int s = smax; s >= 0; --s) {
-    for (int mode : {IUWT_RECOMP, IUWT_SMOOTH}) {
+  std::vector<char> issued(count);
+  for (size_t r = 0; r < rounds; ++r) {
+    std::fill(issued.begin(), issued.end(), 0);
+    for (int k0 = 0; k0 < count; ++k0) {
+      if (issued[k0] || r >= state[k0].ops.size()) continue;
+      const Op op0 = state[k0].ops[r];
       calls.clear();
+      pairs.clear();
       who.clear();
-      for (int k = 0; k < count; ++k) {
+      for (int k = k0; k < count; ++k) {
+        State& sk = state[k];
+        if (issued[k] || r >= sk.ops.size() || sk.ops[r].kind != op0.kind || sk.ops[r].s != op0.s) continue;
         const IuwtRecompJob& j = jobs[k];
-        const State& sk = state[k];
-        if (s > sk.s_start) continue;
-        if ((mode == IUWT_RECOMP) != (s >= j.scale_adjust)) continue;
-        IuwtScaleCall c{};
-        c.mode = mode;
-        c.in = sk.cur;
-        c.out_c = sk.remaining == 1 ? j.out : j.work + (size_t)sk.slot * plane;
-        if (mode == IUWT_RECOMP) {
-          c.w_in = j.detail + (size_t)(s - j.scale_adjust) * plane;
-          c.mask = j.mask ? j.mask + (size_t)(s - j.scale_adjust) * plane : nullptr;
+        float* out = r + 1 == sk.ops.size() ? j.out : j.work + (size_t)sk.slot * plane;
+        if (op0.kind == OP_PAIR) {
+          const size_t o0 = (size_t)(op0.s - j.scale_adjust) * plane;
+          PairJob pj{};
+          pj.in = sk.cur;
+          pj.out = out;
+          pj.w_in0 = j.detail + o0;
+          pj.w_in1 = j.detail + o0 + plane;
+          pj.mask0 = j.mask ? j.mask + o0 : nullptr;
+          pj.mask1 = j.mask ? j.mask + o0 + plane : nullptr;
+          pj.pred = j.pred;
+          pairs.push_back(pj);
+        } else {
+          IuwtScaleCall c{};
+          c.mode = op0.kind == OP_RECOMP ? IUWT_RECOMP : IUWT_SMOOTH;
+          c.in = sk.cur;
+          c.out_c = out;
+          if (op0.kind == OP_RECOMP) {
+            c.w_in = j.detail + (size_t)(op0.s - j.scale_adjust) * plane;
+            c.mask = j.mask ? j.mask + (size_t)(op0.s - j.scale_adjust) * plane : nullptr;
+          }
+          c.h = h;
+          c.w = w;
+          c.scale = op0.s;
+          c.pred = j.pred;
+          calls.push_back(c);
         }
-        c.h = h;
-        c.w = w;
-        c.scale = s;
-        c.pred = j.pred;
-        calls.push_back(c);
         who.push_back(k);
-      }
-      if (calls.empty()) continue;
-      iuwt_scale_batch(calls.data(), (int)calls.size(), st);
-      for (size_t i = 0; i < who.size(); ++i) {
-        State& sk = state[who[i]];
-        sk.cur = calls[i].out_c;
+        issued[k] = 1;
+        sk.cur = out;
         sk.slot ^= 1;
-        --sk.remaining;
+      }
+      if (op0.kind == OP_PAIR) {
+        Plan pl;
+        PMS_REQUIRE(recomp2_plan(h, w, op0.s, &pl), "two-scale recomposition plan changed");
+        recomp2_batch(pairs.data(), (int)pairs.size(), h, w, op0.s, pl, st);
+      } else {
+        iuwt_scale_batch(calls.data(), (int)calls.size(), st);
       }
     }
   }
