@@ -153,11 +153,30 @@ enum WalkFlags { F_OUTC = 1, F_OUTW = 2, F_MASK = 4, F_REF = 8 };
 // of memory operations identical.
 __device__ float g_walk_sink[1024];
 
+// Raw buffer access to one plane: loads take a per-lane byte offset (VGPR, fixed for the walk)
+// and a per-row byte offset (SGPR), so a row's loads need no per-lane address arithmetic.  Plane
+// sizes stay below 2^31 bytes (checked on the host); every offset used is inside the plane.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float buf_f32(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ int buf_u8(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  return (int)__builtin_amdgcn_raw_buffer_load_b8(r, (int)voff, (int)soff, 0);
+}
+
+// A walk has at most this many steps: per-step row offsets are computed once, one step per
+// lane of a wave, and read with readlane.
+constexpr int kWalkMaxSteps = 64;
+
 // Row walk with group pipelining over the comb/phase decomposition described at the top of this
 // file; the walk is cut into groups of K rows: while a group's K steps run, the input
 // rows and per-output operands (mask, reference, detail plane) of the next group are already in
 // flight, so the loop body never drains the memory pipe.  Loads never branch (dead slots read a
-// valid column) and stores never branch (no-output steps write to g_walk_sink).
+// valid column) and stores never branch (no-output steps write to g_walk_sink).  Row offsets of
+// every step come from per-lane tables (readlane) and loads are buffer loads with the row offset
+// in a scalar register, which keeps per-step scalar and address arithmetic to a few instructions.
 template <int MODE, int FL, int K, int ML>
 __global__ __launch_bounds__(kThreads) void k_iuwt_walk(ScaleParams sp) {
   constexpr bool OUTC = (MODE != IUWT_DECOMP) || (FL & F_OUTC);
@@ -171,12 +190,13 @@ __global__ __launch_bounds__(kThreads) void k_iuwt_walk(ScaleParams sp) {
   const ScaleJob a = sp.job[blockIdx.y];
   if (a.pred && *a.pred == 0) return;
 
+  // two row buffers of ML*256 slots (every lane writes its slots)
   extern __shared__ float lds[];
+  float* buf0 = lds;
+  float* buf1 = lds + ML * kThreads;
   const int Q = 1 << sp.q_log2;
   const int TJ = sp.tj;
   const int row_elems = (TJ + 2 * H) * Q;
-  float* buf0 = lds;
-  float* buf1 = lds + row_elems;
 
   const unsigned nblk = gridDim.x;
   unsigned b = xcd_remap(blockIdx.x, nblk);
@@ -195,41 +215,38 @@ __global__ __launch_bounds__(kThreads) void k_iuwt_walk(ScaleParams sp) {
   const bool thread_live = j < TJ;
   const int col = cg * Q + q + (tb * TJ + j) * d;
   const bool col_ok = thread_live && col < w;
-  const int col_c = col_ok ? col : 0;
+  const unsigned col_off = col_ok ? 4u * (unsigned)col : 0u;
+  const int tap_lds = thread_live ? (j + H) * Q + q : H * Q;  // dead lanes read valid slots
 
   const int mr = (h - p + d - 1) / d;
   const int o_begin = seg * sp.k_rows;
   const int o_end = min(o_begin + sp.k_rows, mr);
+  const int n_out = o_end - o_begin;
   const int i_begin = o_begin - H;
-  const int n_steps = (o_end - o_begin) + 2 * H;
+  const int n_steps = n_out + 2 * H;  // <= kWalkMaxSteps (plan)
   const int n_groups = (n_steps + K - 1) / K;
 
-  int ld_col[ML];
-  bool ld_ok[ML];
+  // per-step row byte offsets, step = lane: input rows clamped to the walk (the prefetch of the two
+  // groups past the end re-reads the last row, so every index stays within the extension's range),
+  // output rows clamped to a real row for steps that produce none
+  const int lane = t & 63;
+  const int tab_in = fold_idx(p + (i_begin + min(lane, n_steps - 1)) * d, h) * w * 4;
+  const int tab_out = (p + min(max(i_begin + lane - H, o_begin), o_end - 1) * d) * w * 4;
+  auto step_row = [](int tab, int s) { return (unsigned)__builtin_amdgcn_readlane(tab, min(s, kWalkMaxSteps - 1)); };
+
+  unsigned ld_off[ML];
 #pragma unroll
   for (int m = 0; m < ML; ++m) {
     const int e = t + m * kThreads;
-    ld_ok[m] = e < row_elems;
     const int jj = e >> sp.q_log2;
     const int qq = e & (Q - 1);
-    ld_col[m] = ld_ok[m] ? fold_idx(cg * Q + qq + (tb * TJ + jj - H) * d, w) : 0;
+    ld_off[m] = e < row_elems ? 4u * (unsigned)fold_idx(cg * Q + qq + (tb * TJ + jj - H) * d, w) : 0u;
   }
-  // input row of step s; the prefetch of the two groups past the walk's end re-reads the last
-  // row, so every row index stays within the extension's range (at d = h/2 the padding steps
-  // would otherwise reach beyond two periods)
-  auto row_of = [&](int s) {
-    return fold_idx(p + (i_begin + min(s, n_steps - 1)) * d, h);
-  };
-  // output pixel of step s (clamped to a real pixel when the step has none)
-  auto out_off = [&](int s) {
-    int o = i_begin + s - H;
-    o = o < o_begin ? o_begin : (o >= o_end ? o_end - 1 : o);
-    return (size_t)(p + o * d) * w + col_c;
-  };
-  auto has_out = [&](int s) {
-    const int o = i_begin + s - H;
-    return s >= 2 * H && o < o_end && col_ok;
-  };
+  const unsigned plane = (unsigned)h * (unsigned)w;
+  const auto r_in = plane_rsrc(a.in, 4u * plane);
+  const auto r_win = plane_rsrc(a.w_in, 4u * plane);
+  const auto r_ref = plane_rsrc(a.ref, 4u * plane);
+  const auto r_mask = plane_rsrc(a.mask, plane);
 
   struct Group {
     float row[K][ML];
@@ -241,14 +258,14 @@ __global__ __launch_bounds__(kThreads) void k_iuwt_walk(ScaleParams sp) {
 #pragma unroll
     for (int u = 0; u < K; ++u) {
       const int s = g * K + u;
-      const float* src = a.in + (size_t)row_of(s) * w;
+      const unsigned s_in = step_row(tab_in, s);
 #pragma unroll
-      for (int m = 0; m < ML; ++m) G.row[u][m] = src[ld_col[m]];
+      for (int m = 0; m < ML; ++m) G.row[u][m] = buf_f32(r_in, ld_off[m], s_in);
       if constexpr (MASK || REF || WIN) {
-        const size_t off = out_off(s);
-        if constexpr (MASK) G.msk[u] = a.mask[off];
-        if constexpr (REF) G.ref[u] = a.ref[off];
-        if constexpr (WIN) G.win[u] = a.w_in[off];
+        const unsigned s_o = step_row(tab_out, s);
+        if constexpr (MASK) G.msk[u] = buf_u8(r_mask, col_off >> 2, s_o >> 2);
+        if constexpr (REF) G.ref[u] = buf_f32(r_ref, col_off, s_o);
+        if constexpr (WIN) G.win[u] = buf_f32(r_win, col_off, s_o);
       }
     }
   };
@@ -267,19 +284,16 @@ __global__ __launch_bounds__(kThreads) void k_iuwt_walk(ScaleParams sp) {
   auto step = [&](const Group& G, int u, int s) {
     float* cur = (u & 1) ? buf1 : buf0;
 #pragma unroll
-    for (int m = 0; m < ML; ++m)
-      if (ld_ok[m]) cur[t + m * kThreads] = G.row[u][m];
+    for (int m = 0; m < ML; ++m) cur[t + m * kThreads] = G.row[u][m];
     __syncthreads();
-    const float* rowp = cur + (j + H) * Q + q;
+    const float* rowp = cur + tap_lds;
     float ha = 0.f, hb = 0.f, hx = 0.f;
-    if (thread_live) {
 #pragma unroll
-      for (int k = -2; k <= 2; ++k) ha = fmaf(c_h5[k + 2], rowp[k * Q], ha);
-      if (MODE == IUWT_DECOMP) {
+    for (int k = -2; k <= 2; ++k) ha = fmaf(c_h5[k + 2], rowp[k * Q], ha);
+    if (MODE == IUWT_DECOMP) {
 #pragma unroll
-        for (int k = -4; k <= 4; ++k) hb = fmaf(c_h9[k + 4], rowp[k * Q], hb);
-        hx = rowp[0];
-      }
+      for (int k = -4; k <= 4; ++k) hb = fmaf(c_h9[k + 4], rowp[k * Q], hb);
+      hx = rowp[0];
     }
 #pragma unroll
     for (int k = 0; k < WA - 1; ++k) wa[k] = wa[k + 1];
@@ -292,29 +306,32 @@ __global__ __launch_bounds__(kThreads) void k_iuwt_walk(ScaleParams sp) {
       for (int k = 0; k < 4; ++k) wx[k] = wx[k + 1];
       wx[4] = hx;
     }
-    const bool out = has_out(s);
-    const size_t off = out_off(s);
+    const bool out = (unsigned)(s - 2 * H) < (unsigned)n_out && col_ok;
+    const unsigned off = step_row(tab_out, s) + col_off;
     float c = 0.f;
 #pragma unroll
     for (int k = 0; k < 5; ++k) c = fmaf(c_h5[k], wa[k], c);
+    auto dst = [&](float* base) {
+      return out ? reinterpret_cast<float*>(reinterpret_cast<char*>(base) + off) : g_walk_sink + t;
+    };
     if constexpr (MODE == IUWT_DECOMP) {
       float c1 = 0.f;
 #pragma unroll
       for (int k = 0; k < 9; ++k) c1 = fmaf(c_h9[k], wb[k], c1);
       float wv = wx[0] - c1;
       if constexpr (MASK) wv = G.msk[u] == 0 ? 0.f : wv;
-      if constexpr (OUTC) *(out ? a.out_c + off : g_walk_sink + t) = c;
-      if constexpr (OUTW) *(out ? a.out_w + off : g_walk_sink + t) = wv;
+      if constexpr (OUTC) *dst(a.out_c) = c;
+      if constexpr (OUTW) *dst(a.out_w) = wv;
       if constexpr (REF) {
         const float df = out ? G.ref[u] - wv : 0.f;
         acc = fma((double)df, (double)df, acc);
       }
     } else if constexpr (MODE == IUWT_SMOOTH) {
-      *(out ? a.out_c + off : g_walk_sink + t) = c;
+      *dst(a.out_c) = c;
     } else {
       float wi = G.win[u];
       if constexpr (MASK) wi = G.msk[u] == 0 ? 0.f : wi;
-      *(out ? a.out_c + off : g_walk_sink + t) = c + wi;
+      *dst(a.out_c) = c + wi;
     }
   };
 
@@ -362,19 +379,6 @@ __device__ __forceinline__ bool fold_mirrors(int i, int n) {
   return i >= n;
 }
 
-// Raw buffer access to one plane: loads take a per-lane byte offset (VGPR, fixed for the walk)
-// and a per-row byte offset (SGPR), so a row's loads need no per-lane address arithmetic.  Plane
-// sizes stay below 2^31 bytes (checked on the host); every offset used is inside the plane.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* base, unsigned bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
-}
-__device__ __forceinline__ float buf_f32(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
-}
-__device__ __forceinline__ int buf_u8(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-  return (int)__builtin_amdgcn_raw_buffer_load_b8(r, (int)voff, (int)soff, 0);
-}
-
 // Two-scale recomposition walk.  Per input row of R (staged in LDS with a halo of 6 teeth) each
 // thread computes the intermediate T = h_{s+1} R + w_{s+1} for one T lane (teeth -2 .. TJ+1 of
 // the comb: the plan keeps (TJ+4)*Q <= NT, the block size), staged in a second LDS row, and one step later
@@ -389,7 +393,6 @@ __device__ __forceinline__ int buf_u8(__amdgpu_buffer_rsrc_t r, unsigned voff, u
 // T(x) = T(fold(x)) in exact arithmetic, and the rounded sums agree too when the operands are
 // accumulated in the same order: through a mirror the neighbours of x appear in reverse order, so
 // mirrored lanes (columns) and mirrored rows accumulate their 5 taps from the other end.
-constexpr int kPairMaxSteps = 64;
 
 template <int FL, int K, int ML, int NT>
 __global__ __launch_bounds__(NT) void k_iuwt_recomp2(PairParams sp) {
@@ -444,7 +447,7 @@ __global__ __launch_bounds__(NT) void k_iuwt_recomp2(PairParams sp) {
   const int o_end = min(o_begin + sp.k_rows, mr);
   const int n_out = o_end - o_begin;
   const int i_begin = o_begin - LAG_O;
-  const int n_steps = n_out + 2 * LAG_O;  // <= kPairMaxSteps (plan)
+  const int n_steps = n_out + 2 * LAG_O;  // <= kWalkMaxSteps (plan)
   // one more step than rows: stage 2 of the last row runs in it
   const int n_groups = (n_steps + 1 + K - 1) / K;
 
@@ -456,7 +459,7 @@ __global__ __launch_bounds__(NT) void k_iuwt_recomp2(PairParams sp) {
   const int tab_t = fold_idx(p + (i_begin + max(ls, 2 * LAG_T) - LAG_T) * d, h) * w * 4;
   const int tab_out = (p + min(max(i_begin + lane - LAG_O, o_begin), o_end - 1) * d) * w * 4;
   const unsigned long long row_mir = __ballot(fold_mirrors(p + (i_begin + lane - LAG_T) * d, h));
-  auto step_row = [](int tab, int s) { return (unsigned)__builtin_amdgcn_readlane(tab, min(s, kPairMaxSteps - 1)); };
+  auto step_row = [](int tab, int s) { return (unsigned)__builtin_amdgcn_readlane(tab, min(s, kWalkMaxSteps - 1)); };
 
   unsigned ld_off[ML];
 #pragma unroll
@@ -631,13 +634,16 @@ Plan make_plan(int h, int w, int scale, int halo) {
     const int kmin = 4 * halo;
     if (k < kmin) k = kmin;
   }
+  if (k > kWalkMaxSteps - 2 * halo) k = kWalkMaxSteps - 2 * halo;  // a walk's steps fit one wave's lanes
   if (k > mr) k = mr;
   pl.k_rows = k;
   pl.n_cg = ceil_div(d, q);
   pl.n_tb = ceil_div(teeth, tj);
   pl.n_seg = ceil_div(mr, k);
   pl.blocks = d * pl.n_cg * pl.n_tb * pl.n_seg;
-  pl.lds_bytes = 2 * (tj + 2 * halo) * q * (int)sizeof(float);
+  const int elems = (tj + 2 * halo) * q;
+  const int ml = elems <= 2 * kThreads ? 2 : (elems <= 4 * kThreads ? 4 : 8);
+  pl.lds_bytes = 2 * ml * kThreads * (int)sizeof(float);
   return pl;
 }
 
@@ -710,7 +716,7 @@ bool recomp2_plan(int h, int w, int s, Plan* out) {
     k = ceil_div(mr, (int)segs);
     if (k < 24) k = 24;
   }
-  if (k > kPairMaxSteps - 12) k = kPairMaxSteps - 12;  // a walk's steps fit one wave's lanes
+  if (k > kWalkMaxSteps - 12) k = kWalkMaxSteps - 12;  // a walk's steps fit one wave's lanes
   if (k > mr) k = mr;
   pl.k_rows = k;
   pl.n_cg = ceil_div(d, q);
@@ -789,6 +795,7 @@ int walk_flags(const IuwtScaleCall& c) {
 void check_iuwt_scale(int h, int w, int scale) {
   PMS_REQUIRE(h >= 2 && w >= 2, "IUWT planes must be at least 2x2");
   PMS_REQUIRE(scale >= 0 && scale < 30, "IUWT scale out of range");
+  PMS_REQUIRE((long)h * w < (1L << 29), "IUWT planes are limited to 2^29 pixels (32-bit byte offsets)");
   PMS_REQUIRE((2L << scale) <= h && (2L << scale) <= w,
               "IUWT scale " + std::to_string(scale) + " needs 2^(scale+1) <= image side (" +
                   std::to_string(h) + "x" + std::to_string(w) + ")");

@@ -26,8 +26,11 @@ def say(msg):
 
 
 def rel(a, b):
-    a = np.asarray(a, dtype=np.float64)
-    b = np.asarray(b, dtype=np.float64)
+    a = np.asarray(a)
+    b = np.asarray(b)
+    dt = np.complex128 if np.iscomplexobj(a) or np.iscomplexobj(b) else np.float64
+    a = a.astype(dt)
+    b = b.astype(dt)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
