@@ -321,7 +321,9 @@ __global__ __launch_bounds__(kThreads) void k_iuwt_walk(ScaleParams sp) {
       float wv = wx[0] - c1;
       if constexpr (MASK) wv = G.msk[u] == 0 ? 0.f : wv;
       if constexpr (OUTC) *dst(a.out_c) = c;
-      if constexpr (OUTW) *dst(a.out_w) = wv;
+      // detail planes are not read again by the walk: non-temporal stores keep the smoothed
+      // planes (re-read at the next scale) in the caches
+      if constexpr (OUTW) __builtin_nontemporal_store(wv, dst(a.out_w));
       if constexpr (REF) {
         const float df = out ? G.ref[u] - wv : 0.f;
         acc = fma((double)df, (double)df, acc);
