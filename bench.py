@@ -35,6 +35,21 @@ def log(msg):
     sys.stderr.flush()
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "c1_hbm_traffic.json")
+
+
+def measured_traffic(op, workload):
+    """HBM bytes per operation from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this
+    workload (copy-calibrated), or None when there is none for this configuration."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            doc = json.load(f)
+    except (OSError, ValueError):
+        return None
+    rec = doc.get("ops", {}).get(op) if doc.get("workload") == workload else None
+    return int(round(rec["traffic_bytes"])) if rec else None
+
+
 def gbs(nbytes, ms):
     return nbytes / (ms * 1e-3) / 1e9
 
@@ -279,12 +294,15 @@ def main():
 
     plane = 4 * n * n
     b_iuwt = (S + 2) * plane
+    # row-walk planes recompose detail scales two per launch, paired from the finest up
+    # (at this size every pair qualifies)
+    n_pairs = S // 2 if n * n > 256 * 256 and os.environ.get("PMS_IUWT_PAIR", "2") != "0" else 0
     b_conv = 8 * n * n + 3 * 8 * p * (p // 2 + 1)
     ops = {
         "iuwt_decompose": {"ms": round(ms_dec, 4), "alg_bytes": b_iuwt, "GBps": round(gbs(b_iuwt, ms_dec), 1),
                            "launches": S},
         "iuwt_recompose": {"ms": round(ms_rec, 4), "alg_bytes": b_iuwt, "GBps": round(gbs(b_iuwt, ms_rec), 1),
-                           "launches": S},
+                           "launches": S - n_pairs},
         "fft_convolve": {"ms": round(ms_conv, 4), "alg_bytes": b_conv, "GBps": round(gbs(b_conv, ms_conv), 1),
                          "launches": 3},
         "psf_spectrum_once": {"ms": round(spec_ms, 4)},
@@ -308,7 +326,9 @@ def main():
                                "linear fft_convolve with %dx%d PSF spectrum" % (n, n, S, p, p),
                    "image": n, "scale_count": S, "psf": p, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": dom["GBps"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(dom["GBps"] / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(dom["GBps"] / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(dom_name, {"image": n, "scale_count": S, "psf": p}),
+                     "alg_bytes": dom["alg_bytes"], "launches": dom["launches"]},
         "ops": ops,
         "wall_ms_per_step": round(wall_ms, 4),
         "recon_rel_err": recon_err,

@@ -1,0 +1,159 @@
+"""Summaries of rocprofv3 CSV output for the committed measurement files.
+
+Kernel trace (``--kernel-trace --stats``)::
+
+    python -m pymoresane_amd.prof_summary trace DIR/NAME_kernel_trace.csv DIR/NAME_kernel_stats.csv \
+        --title TITLE --command "rocprofv3 ..." [--notes FILE] > SUMMARY.md
+
+HBM traffic (one ``--pmc FETCH_SIZE`` and one ``--pmc WRITE_SIZE`` run of the workload, and the same two
+of the plain copy kernel of known size for the counter calibration)::
+
+    python -m pymoresane_amd.prof_summary traffic FETCH.csv WRITE.csv COPY_FETCH.csv COPY_WRITE.csv \
+        --copy-bytes BYTES --op NAME=REGEX=LAUNCHES_PER_OP ... [--bench BENCH.json] \
+        [--workload-key JSON] [--json OUT.json] --commands FILE > SUMMARY.md
+
+FETCH_SIZE and WRITE_SIZE are in KiB per dispatch; each is scaled by (true bytes of the copy) / (counter
+of the copy), so the figures are bytes as the plain copy kernel counts them.
+"""
+
+import argparse
+import collections
+import csv
+import json
+import re
+import sys
+
+
+def short_name(name):
+    """'void pms::(anonymous namespace)::k_conv_cols<13, true>(...)' -> 'k_conv_cols<13, true>'."""
+    m = re.search(r"(k_\w+(<[^()]*>)?)\(", name)
+    return m.group(1) if m else name[:80]
+
+
+def load_counters(path, counter):
+    rows = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"].startswith(counter):
+                rows.append((short_name(r["Kernel_Name"]), int(r["Grid_Size"]), float(r["Counter_Value"])))
+    return rows
+
+
+def trace(args):
+    per = collections.defaultdict(list)
+    grid = {}
+    with open(args.kernel_trace) as f:
+        for r in csv.DictReader(f):
+            n = short_name(r["Kernel_Name"])
+            per[n].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1000.0)
+            grid[n] = "%s x %s" % (r["Grid_Size_X"], r.get("Workgroup_Size_X", ""))
+    out = ["# Kernel trace: %s" % args.title, "", "Command (repo root, GPU box, MI355X):", "", "```",
+           args.command, "```", ""]
+    if args.notes:
+        out += [open(args.notes).read().rstrip(), ""]
+    out += ["| kernel | grid threads x block | calls | avg us | min us | max us |", "|---|---|---|---|---|---|"]
+    for n, v in sorted(per.items(), key=lambda kv: -sum(kv[1])):
+        out.append("| %s | %s | %d | %.1f | %.1f | %.1f |" % (n, grid[n], len(v), sum(v) / len(v), min(v), max(v)))
+    out += ["", "rocprofv3 --stats summary (kernel_stats.csv):", "", "| name | calls | avg ns | total ns | % |",
+            "|---|---|---|---|---|"]
+    with open(args.kernel_stats) as f:
+        for r in csv.DictReader(f):
+            out.append("| %s | %s | %.1f | %s | %s |" % (short_name(r["Name"]), r["Calls"], float(r["AverageNs"]),
+                                                         r["TotalDurationNs"], r["Percentage"]))
+    print("\n".join(out))
+
+
+def traffic(args):
+    fetch = load_counters(args.fetch, "FETCH_SIZE")
+    write = load_counters(args.write, "WRITE_SIZE")
+    cf = [v for n, g, v in load_counters(args.copy_fetch, "FETCH_SIZE") if n.startswith("k_copy")]
+    cw = [v for n, g, v in load_counters(args.copy_write, "WRITE_SIZE") if n.startswith("k_copy")]
+    if not cf or not cw:
+        sys.exit("no k_copy dispatches in the calibration runs")
+    kib_f, kib_w = sum(cf) / len(cf), sum(cw) / len(cw)
+    f_scale = args.copy_bytes / (kib_f * 1024.0)
+    w_scale = args.copy_bytes / (kib_w * 1024.0)
+
+    def table(rows):
+        t = collections.defaultdict(list)
+        for n, g, v in rows:
+            t[(n, g)].append(v)
+        return t
+
+    tf, tw = table(fetch), table(write)
+    out = ["# HBM traffic per launch: %s" % args.title, "", "Commands (repo root, GPU box), one counter per pass:", "",
+           "```", open(args.commands).read().rstrip(), "```", "",
+           "Calibration: the plain copy kernel (k_copy) moving %d bytes each way reads FETCH_SIZE = %.1f KiB and "
+           "WRITE_SIZE = %.1f KiB per dispatch, so the scale factors to true bytes are **fetch x%.3f** and "
+           "**write x%.3f**." % (args.copy_bytes, kib_f, kib_w, f_scale, w_scale), "",
+           "| kernel | grid | dispatches | FETCH KiB (raw avg) | WRITE KiB (raw avg) | traffic MB (calibrated) |",
+           "|---|---|---|---|---|---|"]
+    per_kernel = {}
+    for key in sorted(set(tf) | set(tw), key=lambda k: -sum(tf.get(k, [0])) - sum(tw.get(k, [0]))):
+        f = tf.get(key, [0.0])
+        w = tw.get(key, [0.0])
+        fb = sum(f) / len(f) * 1024 * f_scale
+        wb = sum(w) / len(w) * 1024 * w_scale
+        per_kernel[key] = (len(f), sum(f) * 1024 * f_scale, sum(w) * 1024 * w_scale)
+        out.append("| %s | %d | %d | %.0f | %.0f | %.1f |" % (key[0], key[1], len(f), sum(f) / len(f), sum(w) / len(w),
+                                                             (fb + wb) / 1e6))
+    bench_ops = {}
+    if args.bench:
+        with open(args.bench) as fb_:
+            bench_ops = json.loads(fb_.read().strip().splitlines()[-1]).get("ops", {})
+    ops = {}
+    out += ["", "Per operation (kernels of the operation summed, divided by the operation count):", ""]
+    for spec in args.op:
+        name, regex, per_op = spec.split("=")
+        per_op = int(per_op)
+        sel = [k for k in per_kernel if re.search(regex, k[0])]
+        n_disp = sum(per_kernel[k][0] for k in sel)
+        fbytes = sum(per_kernel[k][1] for k in sel)
+        wbytes = sum(per_kernel[k][2] for k in sel)
+        n_ops = n_disp / per_op
+        rec = {"kernels": sorted(set(k[0] for k in sel)), "launches_per_op": per_op,
+               "fetch_bytes": fbytes / n_ops, "write_bytes": wbytes / n_ops,
+               "traffic_bytes": (fbytes + wbytes) / n_ops}
+        line = "* %s (%s; %d launches per operation): fetch %.1f MB + write %.1f MB = **%.1f MB** per operation" % (
+            name, ", ".join(rec["kernels"]), per_op, rec["fetch_bytes"] / 1e6, rec["write_bytes"] / 1e6,
+            rec["traffic_bytes"] / 1e6)
+        if name in bench_ops and "alg_bytes" in bench_ops[name]:
+            alg = bench_ops[name]["alg_bytes"]
+            rec["alg_bytes"] = alg
+            line += " vs %.1f MB algorithmic (%.2fx)" % (alg / 1e6, rec["traffic_bytes"] / alg)
+        out.append(line)
+        ops[name] = rec
+    print("\n".join(out))
+    if args.json:
+        doc = {"workload": json.loads(args.workload_key) if args.workload_key else None,
+               "calibration": {"copy_bytes": args.copy_bytes, "fetch_scale": f_scale, "write_scale": w_scale},
+               "ops": ops}
+        with open(args.json, "w") as f:
+            json.dump(doc, f, indent=1)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("trace")
+    t.add_argument("kernel_trace")
+    t.add_argument("kernel_stats")
+    t.add_argument("--title", required=True)
+    t.add_argument("--command", required=True)
+    t.add_argument("--notes")
+    r = sub.add_parser("traffic")
+    for a in ("fetch", "write", "copy_fetch", "copy_write"):
+        r.add_argument(a)
+    r.add_argument("--copy-bytes", type=float, required=True)
+    r.add_argument("--op", action="append", default=[])
+    r.add_argument("--bench")
+    r.add_argument("--title", required=True)
+    r.add_argument("--commands", required=True)
+    r.add_argument("--workload-key")
+    r.add_argument("--json")
+    args = ap.parse_args()
+    trace(args) if args.cmd == "trace" else traffic(args)
+
+
+if __name__ == "__main__":
+    main()

@@ -38,7 +38,7 @@ def rel(a, b):
 @pytest.mark.parametrize("shape,scales,adjust", [
     ((64, 64), 5, 0), ((64, 64), 5, 2), ((48, 80), 4, 1), ((40, 40), 4, 0), ((256, 256), 7, 0),
     ((256, 256), 4, 3), ((96, 32), 4, 0), ((512, 512), 6, 0),
-    # planes above 512^2 pixels take the row-walk kernel
+    # planes above 256^2 pixels take the row-walk kernel
     ((1024, 1024), 5, 0), ((768, 1040), 4, 1), ((1040, 600), 6, 2),
 ])
 def test_iuwt_decomposition_matches_oracle(hip, shape, scales, adjust):
