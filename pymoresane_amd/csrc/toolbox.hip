@@ -159,6 +159,7 @@ __global__ __launch_bounds__(kT) void k_mad_select(SelState* st, uint32_t* hist,
   __shared__ uint32_t part[kT];
   const int nb = 1 << nbits;
   const int per = kBins / kT;  // 8 bins per thread
+  const SelState S0 = st[s];   // read by every thread before the barrier; one thread writes it after
   uint32_t local = 0;
   for (int k = 0; k < per; ++k) {
     const int b = threadIdx.x * per + k;
@@ -167,16 +168,27 @@ __global__ __launch_bounds__(kT) void k_mad_select(SelState* st, uint32_t* hist,
     local += v;
     gh[b] = 0u;
   }
-  part[threadIdx.x] = local;
+  // inclusive scan of the per-thread counts (wave shuffles, then the wave totals)
+  uint32_t inc = local;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t v = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += v;
+  }
+  if (lane == 63) part[wid] = inc;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    SelState S = st[s];
-    uint32_t run = 0;
-    int t = 0;
-    for (; t < kT - 1; ++t) {
-      if (run + part[t] > S.rank) break;
-      run += part[t];
-    }
+  uint32_t before = 0;
+  for (int k = 0; k < wid; ++k) before += part[k];
+  inc += before;
+  const uint32_t exc = inc - local;
+  // the thread whose range of ranks [exc, inc) holds the target rank finishes the search (the
+  // last thread also takes a rank beyond the total, as the serial scan did)
+  const bool mine = (S0.rank >= exc && S0.rank < inc) || (threadIdx.x == kT - 1 && S0.rank >= inc);
+  if (mine) {
+    SelState S = S0;
+    uint32_t run = exc;
+    const int t = threadIdx.x;
     int b = t * per;
     for (; b < t * per + per - 1; ++b) {
       if (run + cnt[b] > S.rank) break;
@@ -214,7 +226,15 @@ __global__ __launch_bounds__(kT) void k_mad_above(const float* __restrict__ plan
     const uint32_t v = __shfl_xor(best, o, 64);
     best = v < best ? v : best;
   }
-  if ((threadIdx.x & 63) == 0) atomicMin(&st[s].hi_key, best);
+  // one atomic per block (per-wave atomics on the one address serialise at the L2)
+  __shared__ uint32_t wmin[kT / 64];
+  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 1; k < kT / 64; ++k) best = wmin[k] < best ? wmin[k] : best;
+    if (best != 0xffffffffu) atomicMin(&st[s].hi_key, best);
+  }
 }
 
 __global__ void k_mad_init(SelState* st, int S, uint32_t n) {
