@@ -518,14 +518,21 @@ struct Stage0 {
   }
 };
 
+// points per thread in the row and column transforms (measured at P = 8192: 32 points per thread
+// beats 16, whose extra threads buy occupancy but not speed)
+#ifndef PMS_FFT_PTS
+#define PMS_FFT_PTS 32
+#endif
+constexpr int kFftPts = PMS_FFT_PTS;
+
 // launch shapes, functions of the frame only
-// Rows: NT threads per row (at most 32 points each, at least one wave), RB rows per block as many as
+// Rows: NT threads per row (at most kFftPts points each, at least one wave), RB rows per block as many as
 // 512 threads, ~140 KB of LDS and >= 256 blocks allow: more rows per block make longer contiguous
 // runs in the transposed (column-major) writes and reads.
 template <int PG, bool LIN>
 struct RowCfg {
   static constexpr int P = 1 << PG, L = P / 2, N = LIN ? L : P;  // N = rows = image side
-  static constexpr int NT = (L / 32 > 64) ? L / 32 : 64;
+  static constexpr int NT = (L / kFftPts > 64) ? L / kFftPts : 64;
   static constexpr int rb() {
     int r = 16;
     while (r > 1 && (r * NT > 512 || (size_t)r * span(L) * sizeof(float2) > 140 * 1024 || N / r < 256))
@@ -549,7 +556,7 @@ struct ColCfg {
     return c;
   }
   static constexpr int CPB = cpb();
-  static constexpr int NT = (P / (256 / CPB) > 32) ? P / 32 : 256 / CPB;
+  static constexpr int NT = (P / (256 / CPB) > kFftPts) ? P / kFftPts : 256 / CPB;
   static constexpr int T = CPB * NT;
   static constexpr size_t LDS = (size_t)CPB * span(P) * sizeof(float2);
 };
