@@ -47,7 +47,7 @@ def trace(args):
             n = short_name(r["Kernel_Name"])
             per[n].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1000.0)
             grid[n] = "%s x %s" % (r["Grid_Size_X"], r.get("Workgroup_Size_X", ""))
-    out = ["# Kernel trace: %s" % args.title, "", "Command (repo root, GPU box, MI355X):", "", "```",
+    out = ["# Kernel trace: %s" % args.title, "", "Command (repo root, on an MI355X):", "", "```",
            args.command, "```", ""]
     if args.notes:
         out += [open(args.notes).read().rstrip(), ""]
@@ -81,7 +81,8 @@ def traffic(args):
         return t
 
     tf, tw = table(fetch), table(write)
-    out = ["# HBM traffic per launch: %s" % args.title, "", "Commands (repo root, GPU box), one counter per pass:", "",
+    out = ["# HBM traffic per launch: %s" % args.title, "",
+           "Commands (repo root, on an MI355X), one counter per pass:", "",
            "```", open(args.commands).read().rstrip(), "```", "",
            "Calibration: the plain copy kernel (k_copy) moving %d bytes each way reads FETCH_SIZE = %.1f KiB and "
            "WRITE_SIZE = %.1f KiB per dispatch, so the scale factors to true bytes are **fetch x%.3f** and "

@@ -3,7 +3,7 @@
     python tests/golden/c2_precision.py ORACLE.npz            # run the oracle (~30 min, CPU)
     python tests/golden/c2_precision.py ORACLE.npz GPU.npz    # compare (oracle file must exist)
 
-GPU.npz is written on the GPU box by ``python bench.py --precision GPU.npz`` (same generator,
+GPU.npz is written on an MI355X by ``python bench.py --precision GPU.npz`` (same generator,
 seed 2: 2048^2 dirty, 4096^2 PSF, 2000 points + 50 blobs, noise 1e-4; moresane_by_scale 1..6
 with loop_gain 0.2, sigma 4, tolerance 0.75, positivity, linear convolution).  The oracle uses
 its pinned fast backends (scipy correlate1d a trous, threaded scipy.fft); with ``--perturb E``
