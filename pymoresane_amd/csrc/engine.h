@@ -85,7 +85,7 @@ class Job {
 
  public:
   // device buffers (sub = subregion side, S = scale capacity)
-  float *dirty_sub = nullptr, *psf_sub = nullptr, *D = nullptr, *Dcg = nullptr, *E = nullptr;
+  float *dirty_sub = nullptr, *psf_sub = nullptr, *D = nullptr, *Dcg = nullptr, *E = nullptr, *MS = nullptr;
   uint8_t* M = nullptr;
   float *rec = nullptr, *x = nullptr, *xn = nullptr, *r = nullptr, *rn = nullptr, *p = nullptr, *ap = nullptr,
         *cbuf = nullptr;
@@ -219,7 +219,7 @@ class Runner {
   Task* task(int slot) { return tasks_[slot].get(); }
   void release(int slot);  // drop a finished task (its workspace is kept)
   bool idle() const;
-  void refresh_options();      // re-read PMS_CG_GRAPH (opt-in graph replay of CG iterations)
+  void refresh_options();      // re-read PMS_CG_GRAPH and PMS_SNR_DIRECT
   void step();                 // one lockstep step (no-op when idle)
   void run();                  // step until every task is done
   hipStream_t stream() const { return st_; }
@@ -243,6 +243,7 @@ class Runner {
   double* upload_host_ = nullptr;  // pinned upload staging, kSlotDoubles per slot
   double wait_seconds_ = 0.0;
   bool use_graph_ = false;
+  bool snr_direct_ = false;
   std::map<std::vector<IterArgs>, hipGraphExec_t> graphs_;
 };
 

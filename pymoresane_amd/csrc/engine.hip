@@ -195,6 +195,52 @@ __global__ __launch_bounds__(kT) void k_cg_p(CgBatch b) {
   }
 }
 
+// Incremental SNR sums.  The SNR of main.py:413-424 compares E with M.W(H xn).  The convolution and
+// the decomposition are linear and xn = x + alpha p (the positivity clip rewrites p so that this
+// holds), so M.W(H xn) = M.W(H x) + alpha M.W(H p), and M.W(H p) is this iteration's apply_op
+// decomposition (Dcg).  The model's masked detail planes MS = M.W(H x) are carried across the
+// iterations of a minor loop (zero at its start, x = 0): MS <- MS + alpha Dcg, D2 = sum (E - MS)^2.
+// This replaces one convolution and one decomposition per iteration by one streaming pass.
+struct SnrJob {
+  float* ms;
+  const float* dcg;
+  const float* e;
+  const double* scal;  // alpha at kAlpha
+  double* partial;
+  size_t n;  // elements: scales x plane (a multiple of 4)
+  int nblk;  // blocks and partials of this job (a function of n only)
+};
+struct SnrBatch {
+  SnrJob j[kMaxBatch];
+};
+
+__global__ __launch_bounds__(kT) void k_cg_snr(SnrBatch b) {
+  const SnrJob& J = b.j[blockIdx.y];
+  if ((int)blockIdx.x >= J.nblk) return;
+  const double a = J.scal[kAlpha];
+  float4* __restrict__ ms = reinterpret_cast<float4*>(J.ms);
+  const float4* __restrict__ dcg = reinterpret_cast<const float4*>(J.dcg);
+  const float4* __restrict__ e = reinterpret_cast<const float4*>(J.e);
+  const size_t n4 = J.n >> 2;
+  double acc = 0.0;
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n4; i += (size_t)J.nblk * kT) {
+    const float4 d = dcg[i], ev = e[i];
+    float4 m = ms[i];
+    m.x = xn_of(a, d.x, m.x);
+    m.y = xn_of(a, d.y, m.y);
+    m.z = xn_of(a, d.z, m.z);
+    m.w = xn_of(a, d.w, m.w);
+    ms[i] = m;
+    const float fx = ev.x - m.x, fy = ev.y - m.y, fz = ev.z - m.z, fw = ev.w - m.w;
+    acc = fma((double)fx, (double)fx, acc);
+    acc = fma((double)fy, (double)fy, acc);
+    acc = fma((double)fz, (double)fz, acc);
+    acc = fma((double)fw, (double)fw, acc);
+  }
+  acc = block_sum_f64(acc);
+  if (threadIdx.x == 0) J.partial[blockIdx.x] = acc;
+}
+
 // model[r0+r, c0+c] += gain * x[r, c]
 __global__ __launch_bounds__(kT) void k_model_add(float* __restrict__ model, int ld, int r0, int c0, int n,
                                                   const float* __restrict__ x, double gain) {
@@ -353,7 +399,7 @@ static JobSizes max_sizes(const JobSizes& a, const JobSizes& b) {
 
 size_t job_footprint_bytes(const Geometry& g) {
   const JobSizes z = job_sizes(g, std::max(g.S, (int)std::log2((double)g.n) - 1));
-  return z.s2 * 10 + z.S_s2 * 3 + z.mask + z.iuwt + z.N2 * 3 + z.spec_sub + z.spec_full + z.fft + z.psf + z.ext +
+  return z.s2 * 10 + z.S_s2 * 4 + z.mask + z.iuwt + z.N2 * 3 + z.spec_sub + z.spec_full + z.fft + z.psf + z.ext +
          z.partial + z.pmax + 64 * 8 + 64 * 4 + 12 * 256;
 }
 
@@ -394,6 +440,7 @@ bool Job::ensure(const Geometry& g, hipStream_t st) {
   D = F(z.S_s2);
   Dcg = F(z.S_s2);
   E = F(z.S_s2);
+  MS = F(z.S_s2);
   M = (uint8_t*)arena_.alloc(z.mask);
   rec = F(z.s2);
   x = F(z.s2);
@@ -536,6 +583,8 @@ void Task::enqueue_post(hipStream_t st) {
         const int ns = max_scale_ - scale_adjust_;
         const float* Es = J.E + (size_t)scale_adjust_ * s2;
         fill_f32(x_, s2, 0.f, st);
+        // the model's masked detail planes of the incremental SNR start at M.W(H x) = 0 (x = 0)
+        fill_f32(J.MS, (size_t)ns * s2, 0.f, st);
         copy_plane(J.rec, r_, s2, nullptr, st);
         copy_plane(J.rec, J.p, s2, nullptr, st);
         dot_partials(r_, nullptr, s2, J.partial, nullptr, st);
@@ -878,6 +927,10 @@ Runner::Runner(int slots, hipStream_t st) : st_(st) {
 void Runner::refresh_options() {
   const char* graph_env = getenv("PMS_CG_GRAPH");
   use_graph_ = graph_env && graph_env[0] == '1';
+  // PMS_SNR_DIRECT=1: SNR sums from a convolution and a decomposition of xn each iteration, as
+  // main.py:413-424 writes them, instead of the incremental update (same quantity by linearity)
+  const char* snr_env = getenv("PMS_SNR_DIRECT");
+  snr_direct_ = snr_env && snr_env[0] == '1';
 }
 
 Runner::~Runner() {
@@ -1099,19 +1152,35 @@ void Runner::enqueue_iterations(const std::vector<IterArgs>& its, const std::vec
   launch_cg_fin(k_cg_beta_fin, cb, nb, st_);
   launch_cg(k_cg_p, cb, nb, st_);
   // model sources and the SNR sums (main.py:413-424)
-  std::vector<FftConvJob> fj;
-  std::vector<IuwtDecompJob> dj;
-  for (int i = 0; i < nb; ++i) {
-    Job& J = *ts[i]->job();
-    const IterArgs& a = its[i];
-    fj.push_back(FftConvJob{a.xn, J.spec_sub, J.cbuf, J.fft_work, nullptr});
-    dj.push_back(IuwtDecompJob{J.cbuf, a.max_scale, a.adj, nullptr, nullptr, J.iuwt_work, J.M + (size_t)a.adj * s2,
-                               J.E + (size_t)a.adj * s2, J.partial, nullptr, 0});
-  }
-  fft_conv_batch(fj.data(), nb, sub, p_sub, st_);
-  iuwt_decompose_batch(dj.data(), nb, sub, sub, st_);
   std::vector<SumJob> sj;
-  for (int i = 0; i < nb; ++i) sj.push_back(SumJob{ts[i]->job()->partial, dj[i].np, ts[i]->scal() + kD2});
+  if (snr_direct_) {
+    std::vector<FftConvJob> fj;
+    std::vector<IuwtDecompJob> dj;
+    for (int i = 0; i < nb; ++i) {
+      Job& J = *ts[i]->job();
+      const IterArgs& a = its[i];
+      fj.push_back(FftConvJob{a.xn, J.spec_sub, J.cbuf, J.fft_work, nullptr});
+      dj.push_back(IuwtDecompJob{J.cbuf, a.max_scale, a.adj, nullptr, nullptr, J.iuwt_work, J.M + (size_t)a.adj * s2,
+                                 J.E + (size_t)a.adj * s2, J.partial, nullptr, 0});
+    }
+    fft_conv_batch(fj.data(), nb, sub, p_sub, st_);
+    iuwt_decompose_batch(dj.data(), nb, sub, sub, st_);
+    for (int i = 0; i < nb; ++i) sj.push_back(SumJob{ts[i]->job()->partial, dj[i].np, ts[i]->scal() + kD2});
+  } else {
+    SnrBatch sb{};
+    int gmax = 1;
+    for (int i = 0; i < nb; ++i) {
+      Job& J = *ts[i]->job();
+      const IterArgs& a = its[i];
+      const size_t n = (size_t)(a.max_scale - a.adj) * s2;
+      const int nblk = cg_grid(n);
+      sb.j[i] = SnrJob{J.MS, J.Dcg, J.E + (size_t)a.adj * s2, ts[i]->scal(), J.partial, n, nblk};
+      gmax = std::max(gmax, nblk);
+      sj.push_back(SumJob{J.partial, nblk, ts[i]->scal() + kD2});
+    }
+    hipLaunchKernelGGL(k_cg_snr, dim3(gmax, nb), dim3(kT), 0, st_, sb);
+    PMS_LAUNCH_CHECK();
+  }
   sum_partials_batch(sj.data(), nb, st_);
 }
 

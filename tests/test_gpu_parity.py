@@ -286,6 +286,37 @@ def run_pair(dirty, psf, **kw):
     return img, ref
 
 
+@pytest.mark.parametrize("case", ["c0", "walk"])
+def test_moresane_incremental_snr_matches_direct(hip, case, monkeypatch):
+    """The minor loop's SNR sums by linearity (M.W(H x) carried across iterations, updated with alpha
+    times this iteration's M.W(H p)) against the direct convolution + decomposition of xn that
+    main.py:413-424 writes: the same decisions, hence the same model and residual bits, and SNR
+    values that agree to float32 rounding."""
+    from pymoresane_amd.main import DataImage
+    if case == "c0":
+        dirty, psf = synth.config_c0()
+        kw = dict(scale_count=4, loop_gain=0.2, sigma_level=4, major_loop_miter=30, tolerance=0.75,
+                  enforce_positivity=True, conv_mode="linear", minor_loop_miter=30, accuracy=1e-6)
+    else:
+        dirty, psf = synth.make_pair(1024, 33, n_points=30, n_blobs=3)
+        kw = dict(scale_count=4, loop_gain=0.2, major_loop_miter=3, minor_loop_miter=12, enforce_positivity=True)
+    say("moresane %s: incremental vs direct SNR" % case)
+    out = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("PMS_SNR_DIRECT", flag)
+        img = DataImage(dirty, psf, None, 1.0, 1.0)
+        img.moresane(**kw)
+        out.append(img)
+    inc, direct = out
+    assert discrete(inc.trace) == discrete(direct.trace)
+    np.testing.assert_array_equal(inc.model, direct.model)
+    np.testing.assert_array_equal(inc.residual, direct.residual)
+    snr_i = [e[4] for e in inc.trace if e[0] == "minor"]
+    snr_d = [e[4] for e in direct.trace if e[0] == "minor"]
+    assert len(snr_i) > 0
+    assert np.allclose(snr_i, snr_d, rtol=1e-5, atol=1e-6), (snr_i, snr_d)
+
+
 def test_moresane_small_linear(hip):
     say("moresane 64^2 linear")
     dirty, psf = synth.make_pair(64, 21, n_points=8, n_blobs=1)
