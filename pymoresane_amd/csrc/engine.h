@@ -85,7 +85,8 @@ class Job {
 
  public:
   // device buffers (sub = subregion side, S = scale capacity)
-  float *dirty_sub = nullptr, *psf_sub = nullptr, *D = nullptr, *Dcg = nullptr, *E = nullptr, *MS = nullptr;
+  float *dirty_sub = nullptr, *psf_sub = nullptr, *D = nullptr, *Dcg = nullptr, *E = nullptr;
+  float* Rs = nullptr;  // E - M.W(H x) of the running minor loop (incremental SNR)
   uint8_t* M = nullptr;
   float *rec = nullptr, *x = nullptr, *xn = nullptr, *r = nullptr, *rn = nullptr, *p = nullptr, *ap = nullptr,
         *cbuf = nullptr;

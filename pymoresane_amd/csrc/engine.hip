@@ -198,13 +198,12 @@ __global__ __launch_bounds__(kT) void k_cg_p(CgBatch b) {
 // Incremental SNR sums.  The SNR of main.py:413-424 compares E with M.W(H xn).  The convolution and
 // the decomposition are linear and xn = x + alpha p (the positivity clip rewrites p so that this
 // holds), so M.W(H xn) = M.W(H x) + alpha M.W(H p), and M.W(H p) is this iteration's apply_op
-// decomposition (Dcg).  The model's masked detail planes MS = M.W(H x) are carried across the
-// iterations of a minor loop (zero at its start, x = 0): MS <- MS + alpha Dcg, D2 = sum (E - MS)^2.
-// This replaces one convolution and one decomposition per iteration by one streaming pass.
+// decomposition (Dcg).  The difference planes R = E - M.W(H x) are carried across the iterations of
+// a minor loop (R = E at its start, x = 0): R <- R - alpha Dcg, D2 = sum R^2.  This replaces one
+// convolution and one decomposition per iteration by one streaming pass over R and Dcg.
 struct SnrJob {
-  float* ms;
+  float* rs;
   const float* dcg;
-  const float* e;
   const double* scal;  // alpha at kAlpha
   double* partial;
   size_t n;  // elements: scales x plane (a multiple of 4)
@@ -218,24 +217,22 @@ __global__ __launch_bounds__(kT) void k_cg_snr(SnrBatch b) {
   const SnrJob& J = b.j[blockIdx.y];
   if ((int)blockIdx.x >= J.nblk) return;
   const double a = J.scal[kAlpha];
-  float4* __restrict__ ms = reinterpret_cast<float4*>(J.ms);
+  float4* __restrict__ rs = reinterpret_cast<float4*>(J.rs);
   const float4* __restrict__ dcg = reinterpret_cast<const float4*>(J.dcg);
-  const float4* __restrict__ e = reinterpret_cast<const float4*>(J.e);
   const size_t n4 = J.n >> 2;
   double acc = 0.0;
   for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n4; i += (size_t)J.nblk * kT) {
-    const float4 d = dcg[i], ev = e[i];
-    float4 m = ms[i];
-    m.x = xn_of(a, d.x, m.x);
-    m.y = xn_of(a, d.y, m.y);
-    m.z = xn_of(a, d.z, m.z);
-    m.w = xn_of(a, d.w, m.w);
-    ms[i] = m;
-    const float fx = ev.x - m.x, fy = ev.y - m.y, fz = ev.z - m.z, fw = ev.w - m.w;
-    acc = fma((double)fx, (double)fx, acc);
-    acc = fma((double)fy, (double)fy, acc);
-    acc = fma((double)fz, (double)fz, acc);
-    acc = fma((double)fw, (double)fw, acc);
+    const float4 d = dcg[i];
+    float4 r = rs[i];
+    r.x = rn_of(a, d.x, r.x);
+    r.y = rn_of(a, d.y, r.y);
+    r.z = rn_of(a, d.z, r.z);
+    r.w = rn_of(a, d.w, r.w);
+    rs[i] = r;
+    acc = fma((double)r.x, (double)r.x, acc);
+    acc = fma((double)r.y, (double)r.y, acc);
+    acc = fma((double)r.z, (double)r.z, acc);
+    acc = fma((double)r.w, (double)r.w, acc);
   }
   acc = block_sum_f64(acc);
   if (threadIdx.x == 0) J.partial[blockIdx.x] = acc;
@@ -440,7 +437,7 @@ bool Job::ensure(const Geometry& g, hipStream_t st) {
   D = F(z.S_s2);
   Dcg = F(z.S_s2);
   E = F(z.S_s2);
-  MS = F(z.S_s2);
+  Rs = F(z.S_s2);
   M = (uint8_t*)arena_.alloc(z.mask);
   rec = F(z.s2);
   x = F(z.s2);
@@ -583,8 +580,8 @@ void Task::enqueue_post(hipStream_t st) {
         const int ns = max_scale_ - scale_adjust_;
         const float* Es = J.E + (size_t)scale_adjust_ * s2;
         fill_f32(x_, s2, 0.f, st);
-        // the model's masked detail planes of the incremental SNR start at M.W(H x) = 0 (x = 0)
-        fill_f32(J.MS, (size_t)ns * s2, 0.f, st);
+        // the incremental SNR's difference planes start at E - M.W(H x) = E (x = 0)
+        copy_plane(Es, J.Rs, (size_t)ns * s2, nullptr, st);
         copy_plane(J.rec, r_, s2, nullptr, st);
         copy_plane(J.rec, J.p, s2, nullptr, st);
         dot_partials(r_, nullptr, s2, J.partial, nullptr, st);
@@ -1174,7 +1171,7 @@ void Runner::enqueue_iterations(const std::vector<IterArgs>& its, const std::vec
       const IterArgs& a = its[i];
       const size_t n = (size_t)(a.max_scale - a.adj) * s2;
       const int nblk = cg_grid(n);
-      sb.j[i] = SnrJob{J.MS, J.Dcg, J.E + (size_t)a.adj * s2, ts[i]->scal(), J.partial, n, nblk};
+      sb.j[i] = SnrJob{J.Rs, J.Dcg, ts[i]->scal(), J.partial, n, nblk};
       gmax = std::max(gmax, nblk);
       sj.push_back(SumJob{J.partial, nblk, ts[i]->scal() + kD2});
     }
