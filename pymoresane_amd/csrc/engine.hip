@@ -128,14 +128,24 @@ __global__ __launch_bounds__(kT) void k_cg_clip(CgBatch b) {
   const CgJob& J = b.j[blockIdx.y];
   if (J.iflag[kClip] == 0) return;
   const double a = J.scal[kAlpha];
-  float* __restrict__ xn = J.xn;
-  const float* __restrict__ x = J.x;
-  float* __restrict__ p = J.p;
-  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < b.n; i += (size_t)gridDim.x * kT) {
-    float v = xn[i];
+  float4* __restrict__ xn = reinterpret_cast<float4*>(J.xn);
+  const float4* __restrict__ x = reinterpret_cast<const float4*>(J.x);
+  float4* __restrict__ p = reinterpret_cast<float4*>(J.p);
+  const size_t n4 = b.n >> 2;
+  auto clip = [a](float& v, float xv) {
     if (v < 0.f) v = 0.f;
+    return (float)(((double)v - (double)xv) / a);
+  };
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n4; i += (size_t)gridDim.x * kT) {
+    float4 v = xn[i];
+    const float4 u = x[i];
+    float4 q;
+    q.x = clip(v.x, u.x);
+    q.y = clip(v.y, u.y);
+    q.z = clip(v.z, u.z);
+    q.w = clip(v.w, u.w);
     xn[i] = v;
-    p[i] = (float)(((double)v - (double)x[i]) / a);
+    p[i] = q;
   }
 }
 
