@@ -25,6 +25,11 @@
 // detail epilogue optionally multiplies by a {0,1} byte mask and/or reduces
 // sum((E - w)^2) against a reference plane instead of storing w.
 #include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "pms_internal.h"
@@ -611,6 +616,15 @@ const IuwtTuning& tuning() {
   return t;
 }
 
+// Launch parameters an autotuned call runs with (set for the duration of the call on its thread).
+struct TuneParams {
+  int target_blocks;  // workgroups a one-scale walk aims for
+  int pair;           // recomposition pairing (0 off, 1 from the coarsest, 2 from the finest)
+};
+thread_local const TuneParams* t_active = nullptr;
+
+int plan_target_blocks() { return t_active ? t_active->target_blocks : tuning().target_blocks; }
+
 Plan make_plan(int h, int w, int scale, int halo) {
   Plan pl;
   const int d = 1 << scale;
@@ -629,7 +643,7 @@ Plan make_plan(int h, int w, int scale, int halo) {
   const int strips = d * ceil_div(d, q) * ceil_div(teeth, tj);
   int k = tuning().krows;
   if (k <= 0) {
-    const long target = tuning().target_blocks;
+    const long target = plan_target_blocks();
     long segs = (target + strips - 1) / strips;
     if (segs < 1) segs = 1;
     k = ceil_div(mr, (int)segs);
@@ -712,7 +726,7 @@ bool recomp2_plan(int h, int w, int s, Plan* out) {
   const int strips = d * ceil_div(d, q) * ceil_div(teeth, tj);
   int k = tuning().krows;
   if (k <= 0) {
-    const long target = tuning().target_blocks;
+    const long target = plan_target_blocks();
     long segs = (target + strips - 1) / strips;
     if (segs < 1) segs = 1;
     k = ceil_div(mr, (int)segs);
@@ -910,7 +924,7 @@ long iuwt_decompose(const float* in, int h, int w, int scale_count, int scale_ad
 
 // Scale by scale over all jobs: at scale s a job either pre-smooths (s < scale_adjust) or
 // decomposes (scale_adjust <= s < scale_count); jobs of the same kind share one launch.
-void iuwt_decompose_batch(IuwtDecompJob* jobs, int count, int h, int w, hipStream_t st) {
+static void decompose_batch_impl(IuwtDecompJob* jobs, int count, int h, int w, hipStream_t st) {
   const size_t plane = (size_t)h * w;
   struct State {
     const float* cur;
@@ -996,7 +1010,7 @@ void iuwt_recompose(const float* detail, int nplanes, int h, int w, int scale_ad
 // the shape allows it (the same planes result).  Each job's steps form a list; round r issues every
 // job's r-th step, steps of the same kind and scale sharing one launch.  Without a smoothed base R
 // starts at zero, so a job's coarsest step is its (masked) detail plane itself.
-void iuwt_recompose_batch(const IuwtRecompJob* jobs, int count, int h, int w, hipStream_t st) {
+static void recompose_batch_impl(const IuwtRecompJob* jobs, int count, int h, int w, hipStream_t st) {
   const size_t plane = (size_t)h * w;
   enum { OP_RECOMP, OP_SMOOTH, OP_PAIR };
   struct Op {
@@ -1011,7 +1025,7 @@ void iuwt_recompose_batch(const IuwtRecompJob* jobs, int count, int h, int w, hi
   size_t rounds = 0;
   // PMS_IUWT_PAIR=0 keeps every scale in its own launch, 1 pairs detail scales from the coarsest
   // down, 2 from the finest up (read per call; the results are identical)
-  const int pair = env_int("PMS_IUWT_PAIR", 2);
+  const int pair = t_active ? t_active->pair : env_int("PMS_IUWT_PAIR", 2);
   for (int k = 0; k < count; ++k) {
     const IuwtRecompJob& j = jobs[k];
     PMS_REQUIRE(j.nplanes >= 1 && j.scale_adjust >= 0, "recomposition needs at least one detail plane");
@@ -1110,6 +1124,208 @@ void iuwt_recompose_batch(const IuwtRecompJob* jobs, int count, int h, int w, hi
       }
     }
   }
+}
+
+// ---- autotuner ------------------------------------------------------------------------------
+// The row walks' launch shape (target workgroup count) and the recomposition's scale pairing change
+// only how the work is cut, never a pixel's arithmetic, so every candidate gives the same planes.
+// On the first call of a (shape, scales, operands, batch size) signature the candidates run on the
+// call's own buffers (pure functions of their inputs) and are timed with events; the fastest is
+// kept in a per-process table (mutex-guarded) and, with PMS_AUTOTUNE_CACHE=<file>, in a JSON file
+// keyed by device architecture and signature that later processes load.  PMS_AUTOTUNE=0 keeps the
+// defaults.  Calls that reduce against a reference plane keep the default shape (their partial
+// sums are sized by it), and calls under stream capture or with predicated jobs use the table or
+// the defaults without timing.
+namespace {
+
+std::mutex g_tune_mu;
+std::map<std::string, TuneParams> g_tune_table;
+bool g_tune_loaded = false;
+
+const char* tune_cache_path() { return getenv("PMS_AUTOTUNE_CACHE"); }
+
+// architecture name of the current device (looked up once per device; caller holds g_tune_mu)
+std::string device_arch_locked() {
+  static std::map<int, std::string> names;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return "unknown";
+  auto it = names.find(dev);
+  if (it != names.end()) return it->second;
+  hipDeviceProp_t prop;
+  std::string a = hipGetDeviceProperties(&prop, dev) == hipSuccess ? std::string(prop.gcnArchName) : "unknown";
+  const size_t colon = a.find(':');
+  if (colon != std::string::npos) a = a.substr(0, colon);
+  names[dev] = a;
+  return a;
+}
+
+// file format: a JSON object, one entry per line: "<arch>|<signature>": {"target_blocks": N, "pair": P}
+void load_tune_cache_locked() {
+  g_tune_loaded = true;
+  const char* path = tune_cache_path();
+  if (!path) return;
+  FILE* f = fopen(path, "r");
+  if (!f) return;
+  char line[1024];
+  while (fgets(line, sizeof(line), f)) {
+    char key[768];
+    int tb = 0, pr = 0;
+    if (sscanf(line, " \"%767[^\"]\": {\"target_blocks\": %d, \"pair\": %d", key, &tb, &pr) == 3 && tb > 0)
+      g_tune_table[key] = TuneParams{tb, pr};
+  }
+  fclose(f);
+}
+
+void save_tune_cache_locked() {
+  const char* path = tune_cache_path();
+  if (!path) return;
+  const std::string tmp = std::string(path) + ".tmp";
+  FILE* f = fopen(tmp.c_str(), "w");
+  if (!f) return;
+  fprintf(f, "{\n");
+  size_t i = 0;
+  for (const auto& kv : g_tune_table)
+    fprintf(f, "\"%s\": {\"target_blocks\": %d, \"pair\": %d}%s\n", kv.first.c_str(), kv.second.target_blocks,
+            kv.second.pair, ++i < g_tune_table.size() ? "," : "");
+  fprintf(f, "}\n");
+  fclose(f);
+  (void)std::rename(tmp.c_str(), path);
+}
+
+int batch_bucket(int count) {
+  int b = 1;
+  while (b < count) b <<= 1;
+  return b;
+}
+
+struct ActiveGuard {
+  explicit ActiveGuard(const TuneParams* p) { t_active = p; }
+  ~ActiveGuard() { t_active = nullptr; }
+};
+
+// Runs `run` with tuned parameters for signature `sig`; `can_time` says whether the call may be
+// repeated for timing (no stream capture, no predicates, no input/output aliasing).
+template <class Run>
+void autotuned(const std::string& sig, const std::vector<TuneParams>& cands, bool can_time, hipStream_t st,
+               Run run) {
+  const char* en = getenv("PMS_AUTOTUNE");
+  if ((en && en[0] == '0') || cands.size() < 2) {
+    run();
+    return;
+  }
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) can_time = false;
+  std::string key;
+  TuneParams tp{};
+  bool have = false;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    key = device_arch_locked() + "|" + sig;
+    if (!g_tune_loaded) load_tune_cache_locked();
+    auto it = g_tune_table.find(key);
+    if (it != g_tune_table.end()) {
+      tp = it->second;
+      have = true;
+    }
+  }
+  if (!have && can_time) {
+    hipEvent_t e0, e1;
+    PMS_HIP(hipEventCreate(&e0));
+    PMS_HIP(hipEventCreate(&e1));
+    float best = INFINITY;
+    for (const TuneParams& c : cands) {
+      ActiveGuard g(&c);
+      run();  // warm (caches, code objects)
+      for (int rep = 0; rep < 2; ++rep) {
+        PMS_HIP(hipEventRecord(e0, st));
+        run();
+        PMS_HIP(hipEventRecord(e1, st));
+        PMS_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        PMS_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+          best = ms;
+          tp = c;
+        }
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    have = true;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_table[key] = tp;
+    save_tune_cache_locked();
+  }
+  if (have) {
+    ActiveGuard g(&tp);
+    run();
+  } else {
+    run();
+  }
+}
+
+std::vector<TuneParams> walk_candidates(int h, int w, bool recomp) {
+  std::vector<TuneParams> c;
+  if ((long)h * w <= tuning().small_pixels) return c;  // direct-stencil kernel: nothing to tune
+  const bool fixed_blocks = getenv("PMS_IUWT_BLOCKS") != nullptr;
+  const bool fixed_pair = getenv("PMS_IUWT_PAIR") != nullptr;
+  const int pair_default = env_int("PMS_IUWT_PAIR", 2);
+  std::vector<int> blocks = fixed_blocks ? std::vector<int>{tuning().target_blocks} : std::vector<int>{1024, 2048, 4096};
+  std::vector<int> pairs = (recomp && !fixed_pair) ? std::vector<int>{2, 0} : std::vector<int>{pair_default};
+  for (int p : pairs)
+    for (int b : blocks) c.push_back(TuneParams{b, p});
+  return c;
+}
+
+bool overlaps(const void* a, size_t na, const void* b, size_t nb) {
+  if (!a || !b) return false;
+  const char* pa = static_cast<const char*>(a);
+  const char* pb = static_cast<const char*>(b);
+  return pa < pb + nb && pb < pa + na;
+}
+
+}  // namespace
+
+void iuwt_decompose_batch(IuwtDecompJob* jobs, int count, int h, int w, hipStream_t st) {
+  const size_t pb = (size_t)h * w * sizeof(float);
+  bool can_time = true, has_ref = false;
+  std::string sig = "decompose " + std::to_string(h) + "x" + std::to_string(w) + " batch " +
+                    std::to_string(batch_bucket(count));
+  for (int k = 0; k < count; ++k) {
+    const IuwtDecompJob& j = jobs[k];
+    has_ref = has_ref || j.ref;
+    if (j.pred) can_time = false;
+    const size_t nd = (size_t)std::max(0, j.scale_count - j.scale_adjust) * pb;
+    if (overlaps(j.in, pb, j.detail, nd) || overlaps(j.in, pb, j.smoothed, pb) || overlaps(j.in, pb, j.work, 2 * pb))
+      can_time = false;
+    if (k == 0)
+      sig += " S" + std::to_string(j.scale_count) + "/" + std::to_string(j.scale_adjust) + (j.detail ? " w" : "") +
+             (j.smoothed ? " c" : "") + (j.mask ? " m" : "");
+  }
+  if (has_ref) {
+    decompose_batch_impl(jobs, count, h, w, st);
+    return;
+  }
+  autotuned(sig, walk_candidates(h, w, false), can_time, st, [&]() { decompose_batch_impl(jobs, count, h, w, st); });
+}
+
+void iuwt_recompose_batch(const IuwtRecompJob* jobs, int count, int h, int w, hipStream_t st) {
+  const size_t pb = (size_t)h * w * sizeof(float);
+  bool can_time = true;
+  std::string sig = "recompose " + std::to_string(h) + "x" + std::to_string(w) + " batch " +
+                    std::to_string(batch_bucket(count));
+  for (int k = 0; k < count; ++k) {
+    const IuwtRecompJob& j = jobs[k];
+    if (j.pred) can_time = false;
+    const size_t nd = (size_t)std::max(0, j.nplanes) * pb;
+    if (overlaps(j.out, pb, j.detail, nd) || overlaps(j.out, pb, j.smoothed, pb) ||
+        overlaps(j.work, 2 * pb, j.detail, nd) || overlaps(j.work, 2 * pb, j.smoothed, pb))
+      can_time = false;
+    if (k == 0)
+      sig += " S" + std::to_string(j.nplanes) + "/" + std::to_string(j.scale_adjust) + (j.smoothed ? " c" : "") +
+             (j.mask ? " m" : "");
+  }
+  autotuned(sig, walk_candidates(h, w, true), can_time, st, [&]() { recompose_batch_impl(jobs, count, h, w, st); });
 }
 
 }  // namespace pms

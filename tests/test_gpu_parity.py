@@ -86,6 +86,31 @@ def test_iuwt_recomposition_two_scale_launches_are_exact(hip, shape, scales, adj
     assert rel(fused, oiuwt.iuwt_recomposition(det, adjust, sm)) < 1e-5
 
 
+@pytest.mark.parametrize("shape,scales", [((1056, 784), 5), ((2048, 300), 6)])
+def test_iuwt_autotuned_launches_are_exact(hip, shape, scales, monkeypatch, tmp_path):
+    """The autotuner only changes how the walks are cut (workgroup targets, recomposition pairing):
+    tuned and default launches give the same bits, and the choice lands in the JSON cache."""
+    say("iuwt autotune %s S=%d" % (shape, scales))
+    x = np.random.default_rng(6).standard_normal(shape).astype(np.float32)
+    cache = tmp_path / "tune.json"
+    monkeypatch.setenv("PMS_AUTOTUNE_CACHE", str(cache))
+    monkeypatch.setenv("PMS_AUTOTUNE", "1")
+    det_t, sm_t = iuwt.iuwt_decomposition(x, scales, store_smoothed=True)
+    rec_t = iuwt.iuwt_recomposition(det_t, 0, smoothed_array=sm_t)
+    monkeypatch.setenv("PMS_AUTOTUNE", "0")
+    det_d, sm_d = iuwt.iuwt_decomposition(x, scales, store_smoothed=True)
+    rec_d = iuwt.iuwt_recomposition(det_d, 0, smoothed_array=sm_d)
+    np.testing.assert_array_equal(det_t, det_d)
+    np.testing.assert_array_equal(sm_t, sm_d)
+    np.testing.assert_array_equal(rec_t, rec_d)
+    import json
+    entries = json.loads(cache.read_text())
+    tag = "%dx%d" % shape
+    assert any(("decompose " + tag) in k for k in entries) and any(("recompose " + tag) in k for k in entries)
+    for v in entries.values():
+        assert v["target_blocks"] in (1024, 2048, 4096) and v["pair"] in (0, 1, 2)
+
+
 def test_moresane_two_scale_recomposition_is_exact(hip, monkeypatch):
     """Masked recompositions inside the minor loop: the whole deconvolution is bit-identical with and
     without two-scale launches."""
