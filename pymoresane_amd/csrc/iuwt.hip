@@ -24,6 +24,13 @@
 // with period 2N, the two passes compose into one circular filter).  The
 // detail epilogue optionally multiplies by a {0,1} byte mask and/or reduces
 // sum((E - w)^2) against a reference plane instead of storing w.
+//
+// A recomposition handles two consecutive detail scales per launch where the
+// shape allows (k_iuwt_recomp2: the coarser scale is the same filter at
+// spacing 2 in the finer scale's frame), with results bit-identical to one
+// launch per scale.  Planes of at most 256^2 pixels use a direct-stencil
+// kernel instead of the walks.  An autotuner (end of this file) picks the
+// walks' workgroup count and the recomposition pairing per call signature.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
