@@ -54,24 +54,50 @@ def gbs(nbytes, ms):
     return nbytes / (ms * 1e-3) / 1e9
 
 
-def cpu_baseline(sample_n=2048, scales=7):
-    import numpy as np
+def cpu_baseline(img, scales=7, sample_n=2048, psf_fwhm=6.0):
+    """The float64 oracle (one core) on the central sample_n^2 crop of the c1 image: one IUWT
+    decompose + recompose and one linear convolution with a 2*sample_n Gaussian PSF of the c1 FWHM.
+    Returns the timing record and the crop's outputs, which double as the accuracy reference for
+    the GPU's full-size outputs away from the crop border."""
     from oracle import convolution64 as oconv, iuwt64 as oiuwt
     from pymoresane_amd import synth
-    img, _ = synth.config_c1(n=sample_n)
-    psf = synth.make_gaussian_psf(2 * sample_n, 6.0)
+    n = img.shape[0]
+    c0 = (n - sample_n) // 2
+    crop = img[c0:c0 + sample_n, c0:c0 + sample_n]
+    psf = synth.make_gaussian_psf(2 * sample_n, psf_fwhm)
     spec = oconv.psf_spectrum(psf)
     t0 = time.perf_counter()
-    det, sm = oiuwt.iuwt_decomposition(img, scales, store_smoothed=True)
+    det, sm = oiuwt.iuwt_decomposition(crop, scales, store_smoothed=True)
     rec = oiuwt.iuwt_recomposition(det, 0, sm)
-    out = oconv.fft_convolve(img, spec, "linear")
+    out = oconv.fft_convolve(crop, spec, "linear")
     dt = time.perf_counter() - t0
-    del det, sm, rec, out
-    ratio = (4096.0 / sample_n) ** 2
-    return {"value": round(1.0 / dt / ratio, 6), "unit": "images/s", "cores": 1, "kind": "port",
-            "sample": "float64 numpy oracle, one step on a %dx%d crop (IUWT S=%d decompose+recompose + linear "
-                      "fft_convolve P=%d) in %.2f s, scaled by the pixel ratio %.0f to 4096^2 images"
-                      % (sample_n, sample_n, scales, 2 * sample_n, dt, ratio)}
+    del rec
+    ratio = (float(n) / sample_n) ** 2
+    rec_ = {"value": round(1.0 / dt / ratio, 6), "unit": "images/s", "cores": 1, "kind": "port",
+            "sample": "float64 numpy oracle, one step on the central %dx%d crop of the c1 image (IUWT S=%d "
+                      "decompose+recompose + linear fft_convolve P=%d) in %.2f s, scaled by the pixel ratio %.0f "
+                      "to %dx%d images" % (sample_n, sample_n, scales, 2 * sample_n, dt, ratio, n, n)}
+    return rec_, (c0, det, sm, out)
+
+
+def c1_accuracy(ref, det_gpu, sm_gpu, conv_gpu, margin=512):
+    """Relative L2 of the GPU's full-size c1 outputs against the float64 oracle's crop outputs, on
+    the part of the crop at least `margin` pixels from its border (beyond the reach of the crop's
+    own border handling: the IUWT at S=7 reaches 382 pixels, the FWHM-6 Gaussian PSF a few tens)."""
+    import numpy as np
+    c0, det, sm, out = ref
+    m = det.shape[-1]
+    a, b = margin, m - margin
+    g = slice(c0 + a, c0 + b)
+
+    def rel(x, y):
+        x = np.asarray(x, np.float64)
+        return float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
+
+    det_err = [rel(det_gpu[k][g, g], det[k][a:b, a:b]) for k in range(det.shape[0])]
+    return {"vs": "float64 oracle on the central %dx%d crop, compared on its central %dx%d" % (m, m, b - a, b - a),
+            "iuwt_detail_rel_l2_max": max(det_err), "iuwt_smoothed_rel_l2": rel(sm_gpu[g, g], sm[a:b, a:b]),
+            "fft_convolve_rel_l2": rel(conv_gpu[g, g], out[a:b, a:b])}
 
 
 MORESANE_KW = dict(loop_gain=0.2, sigma_level=4, major_loop_miter=100, tolerance=0.75, enforce_positivity=True,
@@ -333,6 +359,7 @@ def main():
         "wall_ms_per_step": round(wall_ms, 4),
         "recon_rel_err": recon_err,
     }
+    gpu_out = None if args.no_cpu else (det.get(), sm.get(), out.get())
     check(lib.pms_stream_destroy(st))
     for buf in (d_img, d_psf, det, sm, rec, out, spec, iw, fw, fw_psf):
         buf.free()
@@ -341,7 +368,9 @@ def main():
         res["served"] = bench_served(args.serve_requests, args.max_batch)
     if not args.no_cpu:
         log("cpu baseline (float64 oracle, 1 core, 2048^2 sample)")
-        res["cpu_baseline"] = cpu_baseline()
+        res["cpu_baseline"], ref = cpu_baseline(img, S)
+        res["c1_error"] = c1_accuracy(ref, *gpu_out)
+        del ref, gpu_out
     print(json.dumps(res))
 
 

@@ -262,3 +262,22 @@ def test_fast_backends_match_restatement():
     finally:
         conv.set_fft_workers(1)
     assert np.max(np.abs(a - b)) < 1e-12
+
+
+def test_c1_crop_interior_matches_full_image():
+    """The c1 accuracy figure compares the full-size GPU outputs with the oracle's outputs on a central
+    crop, away from the crop border: there the crop's IUWT and linear convolution equal the full
+    image's (checked here with the oracle on both sides, at a reduced size)."""
+    import bench
+    from oracle import convolution64 as conv
+    from pymoresane_amd import synth
+    n, S = 512, 4
+    img, _ = synth.config_c1(n=n)
+    psf = synth.make_gaussian_psf(2 * n, 6.0)
+    det, sm = iuwt.iuwt_decomposition(img.astype(np.float64), S, store_smoothed=True)
+    out = conv.fft_convolve(img, conv.psf_spectrum(psf), "linear")
+    _, ref = bench.cpu_baseline(img, S, sample_n=256)
+    err = bench.c1_accuracy(ref, det, sm, out, margin=64)  # S=4 reaches 14 + 32 pixels
+    assert err["iuwt_detail_rel_l2_max"] < 1e-14
+    assert err["iuwt_smoothed_rel_l2"] < 1e-14
+    assert err["fft_convolve_rel_l2"] < 1e-13
