@@ -194,6 +194,27 @@ class DataImage(object):
         data = np.asarray(data)
         _fits.write(name, data.reshape(1, 1, data.shape[0], data.shape[1]), getattr(self, "img_hdr", None))
 
+    def make_logger(self, level="INFO", log_file="PyMORESANE.log"):
+        """This module's logger streaming to stderr and to log_file at `level` (main.py:648-676).
+
+        Handlers added by an earlier call are replaced rather than stacked.
+        """
+        level = getattr(logging, level.upper())
+        logger.setLevel(logging.DEBUG)
+        for h in [h for h in logger.handlers if getattr(h, "_pms_made", False)]:
+            logger.removeHandler(h)
+            h.close()
+        fmt = logging.Formatter("%(asctime)s [%(levelname)s]: %(message)s", datefmt="[%m/%d/%Y] [%I:%M:%S]")
+        handlers = [logging.StreamHandler()]
+        if log_file:
+            handlers.append(logging.FileHandler(log_file, mode="w"))
+        for h in handlers:
+            h.setLevel(level)
+            h.setFormatter(fmt)
+            h._pms_made = True
+            logger.addHandler(h)
+        return logger
+
 
 def _box5(a):
     """5 x 5 box sum with zero padding, same size (mask smoothing, main.py:711-715)."""
@@ -232,3 +253,45 @@ class FitsImage(DataImage):
         self.img_hdr.set("BMIN", float(beam_params[1]))
         self.img_hdr.set("BPA", float(beam_params[2]))
         return beam_params
+
+
+def main(argv=None):
+    """``runsane`` (pymoresane/main.py:736-789): deconvolve a dirty/PSF FITS pair on the GPU, by scale
+    unless ``--singlerun``, and write the model, residual and restored images as FITS.
+
+    Output names are ``OUTPUTNAME_model.fits``, ``_residual.fits`` and ``_restored.fits`` unless
+    ``--modelname`` / ``--residualname`` / ``--restoredname`` give them; without OUTPUTNAME all three
+    must be given (ValueError otherwise).  Returns the FitsImage.
+    """
+    import time
+    from .parser import handle_parser
+    args = handle_parser(argv)
+    if args.outputname is None and (args.residualname is None or args.restoredname is None or
+                                    args.modelname is None):
+        raise ValueError("If outputname is unspecified, residualname, restoredname and modelname must be present.")
+    data = FitsImage(args.dirty, args.psf, args.mask)
+    log = data.make_logger(args.loglevel)
+    log.info("Parameters:\n" + str(args)[10:-1])
+    start = time.time()
+    common = dict(subregion=args.subregion, sigma_level=args.sigmalevel, loop_gain=args.loopgain,
+                  tolerance=args.tolerance, accuracy=args.accuracy, major_loop_miter=args.majorloopmiter,
+                  minor_loop_miter=args.minorloopmiter, all_on_gpu=args.allongpu, decom_mode=args.decommode,
+                  core_count=args.corecount, conv_device=args.convdevice, conv_mode=args.convmode,
+                  extraction_mode=args.extractionmode, enforce_positivity=args.enforcepositivity,
+                  edge_suppression=args.edgesuppression, edge_offset=args.edgeoffset,
+                  flux_threshold=args.fluxthreshold, neg_comp=args.negcomp, edge_excl=args.edgeexcl,
+                  int_excl=args.intexcl)
+    if args.singlerun:
+        data.moresane(scale_count=args.scalecount, **common)
+    else:
+        data.moresane_by_scale(start_scale=args.startscale, stop_scale=args.stopscale, **common)
+    log.info("Elapsed time was %s." % time.strftime("%H:%M:%S", time.gmtime(time.time() - start)))
+    data.save_fits(data.model, args.modelname or args.outputname + "_model.fits")
+    data.save_fits(data.residual, args.residualname or args.outputname + "_residual.fits")
+    data.restore()
+    data.save_fits(data.restored, args.restoredname or args.outputname + "_restored.fits")
+    return data
+
+
+if __name__ == "__main__":
+    main()
