@@ -125,3 +125,28 @@ def test_runsane_tart_end_to_end(hip, tmp_path, monkeypatch):
     assert os.path.exists(str(tmp_path / "PyMORESANE.log"))
     hdr, _ = fits.read(out + "_restored.fits")
     assert "BMAJ" in hdr
+
+
+def test_fitsimage_reads_cubes_and_smooths_the_mask(tmp_path):
+    """FitsImage (main.py:678-733) keeps the RA/DEC plane of (1, 1, N, N) cubes, takes CDELT1 from the PSF
+    header, and turns a mask FITS into mask/max, a 5x5 box filter ('same'), then /max again
+    (main.py:711-715, with its use-before-assignment fixed) -- here against scipy's fftconvolve."""
+    from scipy.signal import fftconvolve
+    rng = np.random.default_rng(8)
+    n = 32
+    h = fits.Header([("CTYPE1", "RA---SIN", ""), ("CTYPE2", "DEC--SIN", ""), ("CTYPE3", "FREQ", ""),
+                     ("CTYPE4", "STOKES", ""), ("CDELT1", 0.25, ""), ("CDELT2", 0.25, "")])
+    dirty = rng.standard_normal((1, 1, n, n)).astype(np.float32)
+    psf = rng.standard_normal((1, 1, 2 * n, 2 * n)).astype(np.float32)
+    mask = (rng.random((1, 1, n, n)) > 0.7).astype(np.float32) * 3.0
+    paths = [str(tmp_path / f) for f in ("d.fits", "p.fits", "m.fits")]
+    for p, a in zip(paths, (dirty, psf, mask)):
+        fits.write(p, a, h)
+    img = pmain.FitsImage(*paths)
+    np.testing.assert_array_equal(img.dirty_data, dirty[0, 0])
+    np.testing.assert_array_equal(img.psf_data, psf[0, 0])
+    assert img.cdelt1 == 0.25 and img.mask_name == paths[2]
+    m = mask[0, 0].astype(np.float64) / mask.max()
+    m = fftconvolve(m, np.ones((5, 5)), mode="same")
+    np.testing.assert_allclose(img.mask, (m / m.max()).astype(np.float32), rtol=0, atol=1e-6)
+    assert pmain.FitsImage(*paths[:2]).mask is None
