@@ -362,6 +362,43 @@ def test_moresane_small_circular_options(hip):
     assert rel(img.residual, ref.residual) < 1e-4
 
 
+@pytest.mark.parametrize("case", ["flux_edge_auto", "subregion_negcomp", "by_scale_from_2"])
+def test_moresane_more_options(hip, case):
+    """Options the other cases leave at their defaults: flux_threshold (shallow deconvolution,
+    main.py:253-255), edge_suppression with an edge_offset, scale_count=None (log2 of the side,
+    main.py:108-109), a subregion without a mask, neg_comp with a lower sigma level and tolerance, and a
+    by-scale run that starts above scale 1: identical trace, model and residual within 1e-4."""
+    say("moresane options: %s" % case)
+    from pymoresane_amd.main import DataImage
+    by_scale = False
+    if case == "flux_edge_auto":
+        dirty, psf = synth.make_pair(64, 26, n_points=10, n_blobs=1)
+        kw = dict(scale_count=None, loop_gain=0.2, major_loop_miter=15, flux_threshold=2.5,
+                  edge_suppression=True, edge_offset=2, enforce_positivity=True)
+    elif case == "subregion_negcomp":
+        dirty, psf = synth.make_pair(128, 27, n_points=12, n_blobs=2)
+        kw = dict(subregion=64, scale_count=3, sigma_level=3, tolerance=0.6, loop_gain=0.2, major_loop_miter=10,
+                  neg_comp=True)
+    else:
+        dirty, psf = synth.make_pair(64, 28, n_points=10, n_blobs=2)
+        kw = dict(start_scale=2, stop_scale=4, loop_gain=0.2, major_loop_miter=10, enforce_positivity=True)
+        by_scale = True
+    img = DataImage(dirty, psf, None, 1.0, 1.0)
+    ref = omain.OracleImage(dirty, psf)
+    if by_scale:
+        img.moresane_by_scale(**kw)
+        ref.moresane_by_scale(**kw)
+    else:
+        img.moresane(**kw)
+        ref.moresane(**kw)
+    assert len(ref.trace) > 0
+    if case == "flux_edge_auto":  # the flux threshold ends the major loop before its iteration cap
+        assert sum(1 for e in ref.trace if e[0] == "update") < kw["major_loop_miter"]
+    assert discrete(img.trace) == discrete(ref.trace)
+    assert rel(img.model, ref.model) < 1e-4
+    assert rel(img.residual, ref.residual) < 1e-4
+
+
 @pytest.mark.parametrize("sub", [None, 64])
 def test_moresane_with_mask(hip, sub):
     """A user mask (main.py:283-285: the decomposition of dirty*mask[subregion] replaces the thresholded
