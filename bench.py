@@ -14,7 +14,10 @@ The CPU baseline is the float64 oracle (single core) on a bounded sample: the sa
 step on a 2048 x 2048 crop (S=7, linear convolution with a 4096 PSF), reported as
 4096^2-equivalent images/s (sample images/s divided by the pixel ratio 4).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Also in the line: the batched (c3) and served (c4) forms, and c0 -- one plain 256^2 deconvolution beside
+the float64 oracle run of it on one core.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch] [--no-c0]
 """
 
 import argparse
@@ -102,6 +105,62 @@ def c1_accuracy(ref, det_gpu, sm_gpu, conv_gpu, margin=512):
 
 MORESANE_KW = dict(loop_gain=0.2, sigma_level=4, major_loop_miter=100, tolerance=0.75, enforce_positivity=True,
                    conv_mode="linear", minor_loop_miter=30, accuracy=1e-6)
+
+
+def one_core():
+    """Context that keeps BLAS/OpenMP pools of the CPU legs at one thread (numpy's FFT, median and
+    scipy's labelling are single-threaded already)."""
+    try:
+        from threadpoolctl import threadpool_limits
+        return threadpool_limits(1)
+    except ImportError:  # pragma: no cover - threadpoolctl ships with the image
+        import contextlib
+        return contextlib.nullcontext()
+
+
+def bench_c0(cpu=True):
+    """c0: one plain MORESANE run (256^2 dirty, 512^2 PSF, S=4, gain 0.2, sigma 4, 100 major, tol 0.75,
+    positivity, linear) on the GPU, timed after one warm run, and -- as its CPU baseline -- the
+    float64 oracle's run of the same deconvolution on one core, with the GPU result's error against it."""
+    import numpy as np
+    from pymoresane_amd import synth
+    from pymoresane_amd.main import DataImage
+    dirty, psf = synth.config_c0()
+    kw = dict(MORESANE_KW, scale_count=4)
+    DataImage(dirty, psf, None, 1.0, 1.0).moresane(**kw)  # warm: code objects, allocations
+    t0 = time.perf_counter()
+    img = DataImage(dirty, psf, None, 1.0, 1.0)
+    img.moresane(**kw)
+    hip_s = time.perf_counter() - t0
+    majors = sum(1 for e in img.trace if e[0] == "update")
+    rec = {"workload": "c0: 256x256 dirty, 512x512 PSF, moresane S=4, gain 0.2, sigma 4, 100 major, tol 0.75, "
+                       "positivity, linear", "hip_s": round(hip_s, 4), "major_iters": majors}
+    if cpu:
+        from oracle import moresane64 as omain
+        ref = omain.OracleImage(dirty, psf)
+        with one_core():
+            t0 = time.perf_counter()
+            ref.moresane(**kw)
+            cpu_s = time.perf_counter() - t0
+
+        def rel(a, b):
+            a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+            return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+        def decisions(trace):  # the discrete part of each trace event
+            out = []
+            for e in trace:
+                e = tuple(e)
+                out.append(e[:5] if e[0] == "scales" else e[:4] if e[0] == "minor" else
+                           (e[0], e[1], bool(e[4])) if e[0] == "update" else e)
+            return out
+
+        rec.update({"cpu_baseline": {"seconds": round(cpu_s, 3), "cores": 1, "kind": "port",
+                                     "sample": "float64 numpy oracle, the same deconvolution"},
+                    "speedup_vs_cpu": round(cpu_s / hip_s, 1), "model_rel_l2": rel(img.model, ref.model),
+                    "residual_rel_l2": rel(img.residual, ref.residual),
+                    "trace_equal": decisions(img.trace) == decisions(ref.trace)})
+    return rec
 
 
 def pct(vals, q):
@@ -218,6 +277,7 @@ def main():
     ap.add_argument("--scales", type=int, default=7)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched (c3) and served (c4) forms")
+    ap.add_argument("--no-c0", action="store_true", help="skip the plain c0 deconvolution and its CPU baseline")
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--max-batch", type=int, default=32, help="jobs per lockstep batch (batched/served forms)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -366,9 +426,13 @@ def main():
     if not args.no_batch:
         res["batched"] = bench_batched(args.max_batch)
         res["served"] = bench_served(args.serve_requests, args.max_batch)
+    if not args.no_c0:
+        log("c0 plain deconvolution%s" % ("" if args.no_cpu else " and its float64 oracle run (1 core)"))
+        res["c0"] = bench_c0(cpu=not args.no_cpu)
     if not args.no_cpu:
         log("cpu baseline (float64 oracle, 1 core, 2048^2 sample)")
-        res["cpu_baseline"], ref = cpu_baseline(img, S)
+        with one_core():
+            res["cpu_baseline"], ref = cpu_baseline(img, S)
         res["c1_error"] = c1_accuracy(ref, *gpu_out)
         del ref, gpu_out
     print(json.dumps(res))
