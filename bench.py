@@ -201,6 +201,24 @@ def bench_batched(max_batch=32, workers=8):
             "batched_job_p50_s": round(pct(lat, 50), 4), "batched_job_p99_s": round(pct(lat, 99), 4)}
 
 
+def bench_c3_class(side, max_batch=32, plain=False):
+    """One size class of c3 alone (its jobs stepped in lockstep on one stream, or one at a time with
+    plain=True): the run that a kernel trace of the batched form profiles."""
+    from pymoresane_amd import batch, synth
+    sides = synth.config_c3_sides()
+    idx = [i for i, s in enumerate(sides) if s == side]
+    if not idx:
+        raise SystemExit("c3 has no %d^2 class (sides: %s)" % (side, sorted(set(sides))))
+    pairs = [synth.config_c3_pair(i, side) for i in idx]
+    jobs = batch.DeviceJobs(pairs, [dict(MORESANE_KW, scale_count=5)] * len(pairs))
+    t0 = time.perf_counter()
+    batch.run_batch(jobs, n_workers=1, max_batch=1 if plain else max_batch)
+    dt = time.perf_counter() - t0
+    return {"c3_class": side, "jobs": len(idx), "form": "plain" if plain else "batched",
+            "max_batch": 1 if plain else max_batch, "seconds": round(dt, 4),
+            "images_per_s": round(len(idx) / dt, 3)}
+
+
 def bench_served(count=60, max_batch=32, rate=20.0, budget=200e9):
     """Poisson arrivals at `rate`/s, sides 512/1024/2048/4096 (50/30/15/5 %), S = log2(side) - 5."""
     import numpy as np
@@ -278,6 +296,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched (c3) and served (c4) forms")
     ap.add_argument("--no-c0", action="store_true", help="skip the plain c0 deconvolution and its CPU baseline")
+    ap.add_argument("--c3-class", type=int, metavar="SIDE",
+                    help="run only the c3 size class SIDE (256/512/1024/2048) batched, print its line, exit")
+    ap.add_argument("--c3-plain", action="store_true", help="with --c3-class: one job at a time")
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--max-batch", type=int, default=32, help="jobs per lockstep batch (batched/served forms)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -310,6 +331,9 @@ def main():
         return
     if args.precision:
         print(json.dumps(run_precision(args.precision)))
+        return
+    if args.c3_class:
+        print(json.dumps(bench_c3_class(args.c3_class, args.max_batch, args.c3_plain)))
         return
     n, S = args.n, args.scales
     p = 2 * n

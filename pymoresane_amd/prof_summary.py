@@ -3,7 +3,7 @@
 Kernel trace (``--kernel-trace --stats``)::
 
     python -m pymoresane_amd.prof_summary trace DIR/NAME_kernel_trace.csv DIR/NAME_kernel_stats.csv \
-        --title TITLE --command "rocprofv3 ..." [--notes FILE] > SUMMARY.md
+        --title TITLE --command "rocprofv3 ..." [--notes FILE] [--group NAME=REGEX ...] > SUMMARY.md
 
 HBM traffic (one ``--pmc FETCH_SIZE`` and one ``--pmc WRITE_SIZE`` run of the workload, and the same two
 of the plain copy kernel of known size for the counter calibration)::
@@ -51,6 +51,19 @@ def trace(args):
            args.command, "```", ""]
     if args.notes:
         out += [open(args.notes).read().rstrip(), ""]
+    if args.group:
+        total = sum(sum(v) for v in per.values())
+        out += ["Kernel time by family (busy time = sum of kernel durations, %.1f ms in all):" % (total / 1e3), "",
+                "| family | kernels | busy ms | share |", "|---|---|---|---|"]
+        left = dict(per)
+        for spec in args.group + ["other=."]:
+            name, regex = spec.split("=", 1)
+            sel = [k for k in left if re.search(regex, k)]
+            t = sum(sum(left.pop(k)) for k in sel)
+            if sel:
+                out.append("| %s | %s | %.2f | %.1f%% |" % (name, ", ".join(sorted(set(re.sub("<.*", "", k) for k in sel))),
+                                                             t / 1e3, 100.0 * t / total))
+        out.append("")
     out += ["| kernel | grid threads x block | calls | avg us | min us | max us |", "|---|---|---|---|---|---|"]
     for n, v in sorted(per.items(), key=lambda kv: -sum(kv[1])):
         out.append("| %s | %s | %d | %.1f | %.1f | %.1f |" % (n, grid[n], len(v), sum(v) / len(v), min(v), max(v)))
@@ -142,6 +155,8 @@ def main():
     t.add_argument("--title", required=True)
     t.add_argument("--command", required=True)
     t.add_argument("--notes")
+    t.add_argument("--group", action="append", default=[],
+                   help="NAME=REGEX: sum the kernels whose short name matches into a family (first match wins)")
     r = sub.add_parser("traffic")
     for a in ("fetch", "write", "copy_fetch", "copy_write"):
         r.add_argument(a)
