@@ -424,6 +424,16 @@ def main():
     for k, v in ops.items():
         if "GBps" in v:
             v["frac_of_8TBps"] = round(v["GBps"] / HBM_PEAK_GBS, 4)
+    # Bytes the launches as structured must move at least (each launch reads its input planes and
+    # writes its outputs once): a one-scale decomposition launch reads C_s and writes C_s+1 and w_s;
+    # a two-scale recomposition launch reads R, w_a, w_b and writes R', a one-scale one 3 planes.
+    # Against these, GBps shows how close the kernels run to the bandwidth; against alg_bytes, what
+    # one launch per scale (or pair) costs relative to a fully fused transform.
+    for k, nb in (("iuwt_decompose", 3 * S * plane),
+                  ("iuwt_recompose", (4 * n_pairs + 3 * (S - 2 * n_pairs)) * plane)):
+        ms_k = ops[k]["ms"]
+        ops[k]["launch_bytes"] = nb
+        ops[k]["launch_GBps"] = round(gbs(nb, ms_k), 1)
     dom_name = max(("iuwt_decompose", "iuwt_recompose", "fft_convolve"), key=lambda k: ops[k]["ms"])
     dom = ops[dom_name]
 
