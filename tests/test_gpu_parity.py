@@ -204,6 +204,25 @@ def test_fft_convolve_full_size_linearity(hip):
     assert rel(cab, ca + cb) < 1e-6
 
 
+@pytest.mark.parametrize("n", [64, 8192])
+def test_fft_convolve_circular_delta_response(hip, n):
+    """Circular mode (P = n) answers a delta at (a, b) with the PSF rolled by (a + n/2, b + n/2) (the
+    circular convolution followed by the reference's fftshift); checked against the oracle at 64 and
+    used at 8192, the largest circular frame."""
+    say("fft_convolve circular delta n=%d" % n)
+    rng = np.random.default_rng(9)
+    psf = rng.random((n, n), dtype=np.float32)
+    x = np.zeros((n, n), np.float32)
+    a, b = n // 3, n // 5 + 1
+    x[a, b] = 1.0
+    expect = np.roll(psf, (a + n // 2, b + n // 2), axis=(0, 1))
+    if n <= 256:
+        ref = oconv.fft_convolve(x, oconv.psf_spectrum(psf), "circular")
+        assert np.max(np.abs(ref - expect)) < 1e-9
+    got = conv.fft_convolve(x, conv.psf_spectrum(psf), conv_mode="circular")
+    assert np.max(np.abs(got - expect)) < 1e-5  # PSF values in [0, 1): a wrong roll or crop errs by O(1)
+
+
 # ------------------------------------------------------------------------------------ threshold
 @pytest.mark.parametrize("shape,ee,ie", [((64, 64), 0, 0), ((63, 64), 0, 0), ((64, 64), 5, 0), ((64, 64), 0, 7),
                                          ((64, 64), 3, 9), ((256, 256), 0, 0), ((1024, 1024), 0, 0)])
