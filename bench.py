@@ -303,6 +303,8 @@ def main():
     ap.add_argument("--c3-class", type=int, metavar="SIDE",
                     help="run only the c3 size class SIDE (256/512/1024/2048) batched, print its line, exit")
     ap.add_argument("--c3-plain", action="store_true", help="with --c3-class: one job at a time")
+    ap.add_argument("--c0-only", action="store_true", help="run only the c0 leg (with its CPU baseline unless "
+                                                            "--no-cpu), print its line, exit")
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--max-batch", type=int, default=32, help="jobs per lockstep batch (batched/served forms)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -338,6 +340,9 @@ def main():
         return
     if args.c3_class:
         print(json.dumps(bench_c3_class(args.c3_class, args.max_batch, args.c3_plain)))
+        return
+    if args.c0_only:
+        print(json.dumps(bench_c0(cpu=not args.no_cpu)))
         return
     n, S = args.n, args.scales
     p = 2 * n
