@@ -61,6 +61,27 @@ __device__ __forceinline__ double block_total(const double* __restrict__ partial
   return block_sum_f64(acc);
 }
 
+// the same totals in every thread of the block (a streaming kernel that needs the previous
+// reduction's result sums its partials itself, in block_total's order, instead of waiting for a
+// one-block finishing launch)
+__device__ __forceinline__ double block_total_all(const double* __restrict__ partial, int np) {
+  __shared__ double s_tot;
+  const double t = block_total(partial, np);
+  if (threadIdx.x == 0) s_tot = t;
+  __syncthreads();
+  return s_tot;
+}
+
+__device__ __forceinline__ float block_min_all(const double* __restrict__ partial, int np) {
+  __shared__ float s_min;
+  float m = INFINITY;
+  for (int i = threadIdx.x; i < np; i += kT) m = fminf(m, (float)partial[i]);
+  m = block_min_f32(m);
+  if (threadIdx.x == 0) s_min = m;
+  __syncthreads();
+  return s_min;
+}
+
 // per-block partials of p . Ap
 __global__ __launch_bounds__(kT) void k_cg_pap(CgBatch b) {
   const CgJob& J = b.j[blockIdx.y];
@@ -114,19 +135,14 @@ __global__ __launch_bounds__(kT) void k_cg_xn(CgBatch b) {
   if (threadIdx.x == 0) J.partial[blockIdx.x] = (double)m;
 }
 
-// clip flag = enforce && min(xn) < 0
-__global__ __launch_bounds__(kT) void k_cg_flag_fin(CgBatch b, int np) {
+// positivity clip when enforce && min(xn) < 0 (the minimum from k_cg_xn's np per-block minima;
+// block 0 stores the flag, which predicates the clip pass's operator launches):
+// xn[xn<0] = 0 ; p = (xn - x) / alpha
+__global__ __launch_bounds__(kT) void k_cg_clip(CgBatch b, int np) {
   const CgJob& J = b.j[blockIdx.y];
-  float m = INFINITY;
-  for (int i = threadIdx.x; i < np; i += kT) m = fminf(m, (float)J.partial[i]);
-  m = block_min_f32(m);
-  if (threadIdx.x == 0) J.iflag[kClip] = (J.enforce && m < 0.f) ? 1 : 0;
-}
-
-// positivity clip: xn[xn<0] = 0 ; p = (xn - x) / alpha
-__global__ __launch_bounds__(kT) void k_cg_clip(CgBatch b) {
-  const CgJob& J = b.j[blockIdx.y];
-  if (J.iflag[kClip] == 0) return;
+  const int flag = (J.enforce && block_min_all(J.partial, np) < 0.f) ? 1 : 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) J.iflag[kClip] = flag;
+  if (flag == 0) return;
   const double a = J.scal[kAlpha];
   float4* __restrict__ xn = reinterpret_cast<float4*>(J.xn);
   const float4* __restrict__ x = reinterpret_cast<const float4*>(J.x);
@@ -177,20 +193,16 @@ __global__ __launch_bounds__(kT) void k_cg_rn(CgBatch b) {
   if (threadIdx.x == 0) J.partial[blockIdx.x] = acc;
 }
 
-// rnrn = rn . rn ; beta = rnrn / rr
-__global__ __launch_bounds__(kT) void k_cg_beta_fin(CgBatch b, int np) {
-  const CgJob& J = b.j[blockIdx.y];
-  const double tot = block_total(J.partial, np);
-  if (threadIdx.x == 0) {
-    *J.rnrn = tot;
-    J.scal[kBeta] = tot / *J.rr;
-  }
-}
-
+// rnrn = rn . rn (from k_cg_rn's np partials) ; beta = rnrn / rr (block 0 stores both) ;
 // p = rn + beta p
-__global__ __launch_bounds__(kT) void k_cg_p(CgBatch b) {
+__global__ __launch_bounds__(kT) void k_cg_p(CgBatch b, int np) {
   const CgJob& J = b.j[blockIdx.y];
-  const double be = J.scal[kBeta];
+  const double tot = block_total_all(J.partial, np);
+  const double be = tot / *J.rr;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *J.rnrn = tot;
+    J.scal[kBeta] = be;
+  }
   const float4* __restrict__ rn = reinterpret_cast<const float4*>(J.rn);
   float4* __restrict__ p = reinterpret_cast<float4*>(J.p);
   const size_t n4 = b.n >> 2;
@@ -298,6 +310,13 @@ void launch_cg(K kernel, const CgBatch& b, int nb, hipStream_t st) {
 template <class K>
 void launch_cg_fin(K kernel, const CgBatch& b, int nb, hipStream_t st) {
   hipLaunchKernelGGL(kernel, dim3(1, nb), dim3(kT), 0, st, b, cg_grid(b.n));
+  PMS_LAUNCH_CHECK();
+}
+
+// a streaming kernel that also finishes the previous kernel's reduction (cg_grid(n) partials)
+template <class K>
+void launch_cg_after(K kernel, const CgBatch& b, int nb, hipStream_t st) {
+  hipLaunchKernelGGL(kernel, dim3(cg_grid(b.n), nb), dim3(kT), 0, st, b, cg_grid(b.n));
   PMS_LAUNCH_CHECK();
 }
 
@@ -1151,13 +1170,11 @@ void Runner::enqueue_iterations(const std::vector<IterArgs>& its, const std::vec
   launch_cg_fin(k_cg_alpha_fin, cb, nb, st_);
   launch_cg(k_cg_xn, cb, nb, st_);
   if (!clip_ts.empty()) {
-    launch_cg_fin(k_cg_flag_fin, clip_b, (int)clip_ts.size(), st_);
-    launch_cg(k_cg_clip, clip_b, (int)clip_ts.size(), st_);
+    launch_cg_after(k_cg_clip, clip_b, (int)clip_ts.size(), st_);
     apply_op(clip_ts, clip_its, true);
   }
   launch_cg(k_cg_rn, cb, nb, st_);
-  launch_cg_fin(k_cg_beta_fin, cb, nb, st_);
-  launch_cg(k_cg_p, cb, nb, st_);
+  launch_cg_after(k_cg_p, cb, nb, st_);
   // model sources and the SNR sums (main.py:413-424)
   std::vector<SumJob> sj;
   if (snr_direct_) {
