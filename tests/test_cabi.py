@@ -4,6 +4,7 @@ import ctypes
 import glob
 import os
 import re
+import sys
 
 import pytest
 
@@ -94,3 +95,12 @@ def test_hip_path_without_gpu_raises_instead_of_falling_back():
     import numpy as np
     with pytest.raises(_lib.PmsError, match="no HIP device"):
         iuwt.iuwt_decomposition(np.zeros((8, 8), np.float32), 2)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus N > 1 exits with status 2 and says so, before any device work."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    res = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                         text=True, timeout=120)
+    assert res.returncode == 2 and "one GPU" in res.stderr and res.stdout == ""
