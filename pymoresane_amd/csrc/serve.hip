@@ -122,11 +122,13 @@ class Server {
   }
 
   void lane_loop(Lane& lane) {
-    if (hipSetDevice(dev_) != hipSuccess) return;
     std::unique_ptr<Runner> rn;
-    try {
+    const int status = run_guarded([&] {
+      PMS_HIP(hipSetDevice(dev_));
       rn = std::make_unique<Runner>(max_batch_);
-    } catch (...) {
+    });
+    if (status != 0) {
+      fail_lane(lane, status);
       return;
     }
     const hipStream_t st = rn->stream();
@@ -196,16 +198,37 @@ class Server {
     }
   }
 
-  void finish(pms_serve_request* r, int status) {
+  void finish(pms_serve_request* r, int status, bool admitted = true) {
     r->t_done = now();
     {
       std::lock_guard<std::mutex> lk(mu_);
-      used_ -= r->footprint;
+      if (admitted) used_ -= r->footprint;
       r->status = status;  // 0 = done, < 0 = error
       --pending_;
     }
     cv_.notify_all();
     done_cv_.notify_all();
+  }
+
+  // A lane whose thread could not set up its device stream and runner fails every request queued
+  // to it (now and until the server stops) with that status, instead of leaving them pending.
+  void fail_lane(Lane& lane, int status) {
+    for (;;) {
+      std::vector<pms_serve_request*> dead;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        lane.free_slots = 0;  // never holds back the requests of other lanes
+        cv_.wait(lk, [&] { return stop_ || !lane.queue.empty(); });
+        if (lane.queue.empty()) break;  // stopping
+        for (auto& q : lane.queue) dead.push_back(q.second);
+        lane.queue.clear();
+      }
+      cv_.notify_all();
+      for (pms_serve_request* r : dead) {
+        r->t_start = now();
+        finish(r, status, false);
+      }
+    }
   }
 
   template <class F>
