@@ -117,7 +117,10 @@ class Server(object):
         return keep
 
     def drain(self):
+        """Wait until every submitted request has finished; the server then drops its references to
+        their host buffers (the caller's dicts from submit() keep the results)."""
         check(self._lib.pms_server_drain(self._h))
+        self._live = [k for k in self._live if k["req"].status > 0]  # > 0: queued or running
 
     def close(self):
         if self._h.value:
