@@ -196,13 +196,22 @@ def bench_batched(max_batch=32, workers=8):
     t_batch = time.perf_counter() - t0
     same = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) for a, b in zip(jobs.results(), ref))
     lat = [s[2] for s in stats]
+    # per size class: the plain run's time on it (jobs back to back) and the span of its jobs in the
+    # batched run (classes overlap there, so the spans do not add up to the whole)
+    classes = {}
+    for side in sorted(set(sides)):
+        ids = [i for i, s in enumerate(sides) if s == side]
+        t0c = min(stats[i][3] for i in ids)
+        classes[str(side)] = {"jobs": len(ids), "plain_s": round(sum(plain_stats[i][2] for i in ids), 3),
+                              "batched_span_s": round(max(stats[i][3] + stats[i][2] for i in ids) - t0c, 3)}
     log("c3: plain %.2fs batched %.2fs (x%.2f) identical=%s" % (t_plain, t_batch, t_plain / t_batch, same))
     return {"workload": "c3: 96 ragged pairs 32x256^2+32x512^2+24x1024^2+8x2048^2, PSF 2N, S=5",
             "plain_images_per_s": round(len(sides) / t_plain, 3),
             "batched_images_per_s": round(len(sides) / t_batch, 3),
             "speedup": round(t_plain / t_batch, 3), "max_batch": max_batch, "class_streams": workers,
             "identical_to_alone": bool(same),
-            "batched_job_p50_s": round(pct(lat, 50), 4), "batched_job_p99_s": round(pct(lat, 99), 4)}
+            "batched_job_p50_s": round(pct(lat, 50), 4), "batched_job_p99_s": round(pct(lat, 99), 4),
+            "classes": classes}
 
 
 def bench_c3_class(side, max_batch=32, plain=False):
