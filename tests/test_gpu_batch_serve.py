@@ -90,6 +90,7 @@ def test_served_mix_equals_alone(hip):
         reqs.append(srv.submit(d, p, **KW))
         time.sleep(0.002)
     srv.drain()
+    assert srv._live == []  # finished requests are released; the caller's dicts keep the results
     for rq, (m0, r0) in zip(reqs, ref):
         assert rq["req"].status == 0
         np.testing.assert_array_equal(rq["model"], m0)
