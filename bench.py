@@ -83,6 +83,48 @@ def cpu_baseline(img, scales=7, sample_n=2048, psf_fwhm=6.0):
     return rec_, (c0, det, sm, out)
 
 
+def cpu_baseline_mp(crop, n, scales=7, psf_fwhm=6.0, workers=None):
+    """The reference's 'mp' CPU mode restated (oracle/mp_iuwt.py, after pymoresane/iuwt.py:193-382:
+    every a trous pass forks `workers` processes over strips of a shared float32 array; the
+    convolution stays serial, as in the reference): one c1 step on the same central crop as the
+    serial baseline, scaled by the same pixel ratio.  It runs in a child Python process that never
+    touches the GPU, since its strip passes fork."""
+    import subprocess
+    import tempfile
+    import numpy as np
+    workers = workers or min(16, len(os.sched_getaffinity(0)))
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "crop.npy")
+        np.save(path, np.ascontiguousarray(crop, dtype=np.float32))
+        res = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-mp-child", path, str(scales),
+                              str(psf_fwhm), str(workers)], capture_output=True, text=True, timeout=900)
+    if res.returncode != 0:
+        raise RuntimeError("mp baseline failed: %s" % res.stderr[-2000:])
+    dt = json.loads(res.stdout.strip().splitlines()[-1])["seconds"]
+    m = crop.shape[0]
+    ratio = (float(n) / m) ** 2
+    return {"value": round(1.0 / dt / ratio, 6), "unit": "images/s", "cores": workers, "kind": "port",
+            "sample": "reference 'mp' mode restated: strip-parallel IUWT (%d forked workers per pass, float32 "
+                      "shared array) + serial float64 linear fft_convolve P=%d, one step on the central %dx%d "
+                      "crop of the c1 image in %.2f s, scaled by the pixel ratio %.0f" % (workers, 2 * m, m, m, dt,
+                                                                                          ratio)}
+
+
+def _cpu_mp_child(path, scales, psf_fwhm, workers):
+    import numpy as np
+    from oracle import convolution64 as oconv, mp_iuwt
+    from pymoresane_amd import synth
+    crop = np.load(path, allow_pickle=False)
+    spec = oconv.psf_spectrum(synth.make_gaussian_psf(2 * crop.shape[0], psf_fwhm))
+    with one_core():
+        t0 = time.perf_counter()
+        det, sm = mp_iuwt.mp_iuwt_decomposition(crop, scales, store_smoothed=True, core_count=workers)
+        mp_iuwt.mp_iuwt_recomposition(det.astype(np.float32), 0, core_count=workers, smoothed_array=sm)
+        oconv.fft_convolve(crop, spec, "linear")
+        dt = time.perf_counter() - t0
+    print(json.dumps({"seconds": dt}))
+
+
 def c1_accuracy(ref, det_gpu, sm_gpu, conv_gpu, margin=512):
     """Relative L2 of the GPU's full-size c1 outputs against the float64 oracle's crop outputs, on
     the part of the crop at least `margin` pixels from its border (beyond the reach of the crop's
@@ -312,6 +354,8 @@ def main():
     ap.add_argument("--c3-class", type=int, metavar="SIDE",
                     help="run only the c3 size class SIDE (256/512/1024/2048) batched, print its line, exit")
     ap.add_argument("--c3-plain", action="store_true", help="with --c3-class: one job at a time")
+    ap.add_argument("--cpu-mp-child", nargs=4, metavar=("CROP_NPY", "SCALES", "FWHM", "WORKERS"),
+                    help=argparse.SUPPRESS)
     ap.add_argument("--c0-only", action="store_true", help="run only the c0 leg (with its CPU baseline unless "
                                                             "--no-cpu), print its line, exit")
     ap.add_argument("--serve-requests", type=int, default=60)
@@ -321,6 +365,10 @@ def main():
     ap.add_argument("--precision", metavar="OUT_NPZ",
                     help="only run the 2048^2 by-scale precision case and write model/residual/trace to OUT_NPZ")
     args = ap.parse_args()
+    if args.cpu_mp_child:  # the mp baseline's child process: CPU only
+        p_, s_, f_, w_ = args.cpu_mp_child
+        _cpu_mp_child(p_, int(s_), float(f_), int(w_))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the MORESANE hot path runs one image per device and this build "
               "has no multi-GPU path; run with --gpus 1", file=sys.stderr)
@@ -485,6 +533,9 @@ def main():
         log("cpu baseline (float64 oracle, 1 core, 2048^2 sample)")
         with one_core():
             res["cpu_baseline"], ref = cpu_baseline(img, S)
+        log("cpu baseline, reference 'mp' mode (strip-parallel IUWT), same sample")
+        c0_ = ref[0]
+        res["cpu_baseline_mp"] = cpu_baseline_mp(img[c0_:c0_ + ref[1].shape[-1], c0_:c0_ + ref[1].shape[-1]], n, S)
         res["c1_error"] = c1_accuracy(ref, *gpu_out)
         del ref, gpu_out
     print(json.dumps(res))

@@ -281,3 +281,16 @@ def test_c1_crop_interior_matches_full_image():
     assert err["iuwt_detail_rel_l2_max"] < 1e-14
     assert err["iuwt_smoothed_rel_l2"] < 1e-14
     assert err["fft_convolve_rel_l2"] < 1e-13
+
+
+def test_mp_restatement_matches_the_serial_oracle():
+    """The strip-parallel 'mp' restatement (timed as the mp CPU baseline) computes the same transform as
+    the serial float64 oracle up to its float32 shared array (iuwt.py:193-382 vs 66-191), including a
+    strip count that does not divide the side."""
+    from oracle import mp_iuwt
+    x = np.random.default_rng(12).standard_normal((60, 60)).astype(np.float32)
+    det, sm = mp_iuwt.mp_iuwt_decomposition(x, 4, 1, store_smoothed=True, core_count=3)
+    rdet, rsm = iuwt.iuwt_decomposition(x, 4, 1, store_smoothed=True)
+    assert np.max(np.abs(det - rdet)) < 1e-5 and np.max(np.abs(sm - rsm)) < 1e-5
+    rec = mp_iuwt.mp_iuwt_recomposition(det.astype(np.float32), 1, core_count=3, smoothed_array=sm)
+    assert np.max(np.abs(rec - iuwt.iuwt_recomposition(rdet, 1, rsm))) < 1e-5
