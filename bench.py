@@ -57,6 +57,20 @@ def gbs(nbytes, ms):
     return nbytes / (ms * 1e-3) / 1e9
 
 
+def host_info():
+    """The CPU the baselines ran on: model name, usable cores, numpy/scipy versions."""
+    import numpy as np
+    import scipy
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), None)
+    except OSError:
+        pass
+    return {"cpu": model, "usable_cores": len(os.sched_getaffinity(0)), "numpy": np.__version__,
+            "scipy": scipy.__version__}
+
+
 def cpu_baseline(img, scales=7, sample_n=2048, psf_fwhm=6.0):
     """The float64 oracle (one core) on the central sample_n^2 crop of the c1 image: one IUWT
     decompose + recompose and one linear convolution with a 2*sample_n Gaussian PSF of the c1 FWHM.
@@ -79,7 +93,8 @@ def cpu_baseline(img, scales=7, sample_n=2048, psf_fwhm=6.0):
     rec_ = {"value": round(1.0 / dt / ratio, 6), "unit": "images/s", "cores": 1, "kind": "port",
             "sample": "float64 numpy oracle, one step on the central %dx%d crop of the c1 image (IUWT S=%d "
                       "decompose+recompose + linear fft_convolve P=%d) in %.2f s, scaled by the pixel ratio %.0f "
-                      "to %dx%d images" % (sample_n, sample_n, scales, 2 * sample_n, dt, ratio, n, n)}
+                      "to %dx%d images" % (sample_n, sample_n, scales, 2 * sample_n, dt, ratio, n, n),
+            "host": host_info()}
     return rec_, (c0, det, sm, out)
 
 
