@@ -204,6 +204,26 @@ def test_fft_convolve_full_size_linearity(hip):
     assert rel(cab, ca + cb) < 1e-6
 
 
+def test_fft_convolve_largest_linear_frame(hip):
+    """8192^2 image, linear mode: the 16384^2 frame, the largest the library takes.  A delta at (u, v)
+    returns the PSF window psf[i - u + N, j - v + N] (the relation the 4096^2 case checks), and the
+    convolution is linear."""
+    say("fft_convolve 8192 linear, P=16384")
+    n = 8192
+    g = np.exp(-0.5 * ((np.arange(2 * n, dtype=np.float32) - n) / 2.5) ** 2).astype(np.float32)
+    psf = np.outer(g, g)
+    spec = conv.psf_spectrum(psf)
+    delta = np.zeros((n, n), np.float32)
+    u, v = 3001, 5000
+    delta[u, v] = 1.0
+    out = conv.fft_convolve(delta, spec, conv_mode="linear")
+    assert np.max(np.abs(out - psf[n - u:2 * n - u, n - v:2 * n - v])) < 2e-6
+    rng = np.random.default_rng(13)
+    a = rng.standard_normal((n, n), dtype=np.float32)
+    b = np.roll(a, 17, axis=0)
+    assert rel(conv.fft_convolve(a + b, spec), conv.fft_convolve(a, spec) + conv.fft_convolve(b, spec)) < 1e-6
+
+
 @pytest.mark.parametrize("n", [64, 8192])
 def test_fft_convolve_circular_delta_response(hip, n):
     """Circular mode (P = n) answers a delta at (a, b) with the PSF rolled by (a + n/2, b + n/2) (the
