@@ -511,6 +511,37 @@ def main():
         ms_k = ops[k]["ms"]
         ops[k]["launch_bytes"] = nb
         ops[k]["launch_GBps"] = round(gbs(nb, ms_k), 1)
+    # The per-major-iteration noise/extraction operations on the same S detail planes (not part of the
+    # step): MAD thresholds (algorithmic: read the planes once), thresholding (read + write), source
+    # extraction of the thresholded planes (read, write the extracted planes and the byte mask).
+    thr = DeviceArray((S,), np.float64)
+    tpl = DeviceArray((S, n, n))
+    ext = DeviceArray((S, n, n))
+    msk = DeviceArray((S, n, n), np.uint8)
+    tw_ = DeviceArray((lib.pms_threshold_work_bytes(S, n, n),), np.uint8)
+    xw_ = DeviceArray((lib.pms_extract_work_bytes(S, n, n),), np.uint8)
+
+    def est():
+        check(lib.pms_estimate_threshold(det.ptr, S, n, n, 0, 0, thr.ptr, tw_.ptr, st))
+
+    def app():
+        check(lib.pms_apply_threshold(det.ptr, S, n, n, thr.ptr, 4.0, tpl.ptr, tw_.ptr, st))
+
+    def extract():
+        check(lib.pms_source_extraction(tpl.ptr, S, n, n, 0.75, 0, ext.ptr, msk.ptr, None, xw_.ptr, st))
+
+    est()
+    app()
+    extract()
+    check(lib.pms_stream_synchronize(st))
+    for name, fn, nb in (("estimate_threshold", est, S * plane), ("apply_threshold", app, 2 * S * plane),
+                         ("source_extraction", extract, 2 * S * plane + S * n * n)):
+        ms_k = timed(fn, k_op)
+        ops[name] = {"ms": round(ms_k, 4), "alg_bytes": nb, "GBps": round(gbs(nb, ms_k), 1),
+                     "frac_of_8TBps": round(gbs(nb, ms_k) / HBM_PEAK_GBS, 4), "step": False}
+    check(lib.pms_stream_synchronize(st))
+    for buf in (thr, tpl, ext, msk, tw_, xw_):
+        buf.free()
     dom_name = max(("iuwt_decompose", "iuwt_recompose", "fft_convolve"), key=lambda k: ops[k]["ms"])
     dom = ops[dom_name]
 
