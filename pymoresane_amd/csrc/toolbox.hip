@@ -134,14 +134,35 @@ __global__ __launch_bounds__(kT) void k_mad_hist(const float* __restrict__ plane
   const float* x = planes + (size_t)s * plane;
   const uint32_t pre = st[s].prefix;
   const uint32_t dmask = (1u << nbits) - 1u;
-  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < plane; i += (size_t)gridDim.x * kT) {
-    if (MASKED) {
-      const int r = (int)(i / w), c = (int)(i % w);
-      if (!in_noise_mask(r, c, inc, exc)) continue;
-    }
-    const uint32_t key = __float_as_uint(x[i]) & 0x7fffffffu;
-    if (prefix_shift < 32 && (key >> prefix_shift) != (pre >> prefix_shift)) continue;
+  auto add = [&](float v) {
+    const uint32_t key = __float_as_uint(v) & 0x7fffffffu;
+    if (prefix_shift < 32 && (key >> prefix_shift) != (pre >> prefix_shift)) return;
     atomicAdd(&h[(key >> shift) & dmask], 1u);
+  };
+  const size_t stride = (size_t)gridDim.x * kT;
+  if (!MASKED && (reinterpret_cast<uintptr_t>(x) & 15) == 0) {
+    // eight keys per thread and iteration, loaded as two float4 before any histogram update, so a
+    // thread has two loads in flight instead of one per atomic (the counts do not depend on order)
+    const float4* x4 = reinterpret_cast<const float4*>(x);
+    const size_t n4 = plane >> 2;
+    size_t i = blockIdx.x * (size_t)kT + threadIdx.x;
+    for (; i + stride < n4; i += 2 * stride) {
+      const float4 a = x4[i], b = x4[i + stride];
+      add(a.x), add(a.y), add(a.z), add(a.w), add(b.x), add(b.y), add(b.z), add(b.w);
+    }
+    if (i < n4) {
+      const float4 a = x4[i];
+      add(a.x), add(a.y), add(a.z), add(a.w);
+    }
+    for (size_t j = 4 * n4 + blockIdx.x * (size_t)kT + threadIdx.x; j < plane; j += stride) add(x[j]);
+  } else {
+    for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < plane; i += stride) {
+      if (MASKED) {
+        const int r = (int)(i / w), c = (int)(i % w);
+        if (!in_noise_mask(r, c, inc, exc)) continue;
+      }
+      add(x[i]);
+    }
   }
   __syncthreads();
   uint32_t* gh = hist + (size_t)s * kBins;
@@ -213,13 +234,27 @@ __global__ __launch_bounds__(kT) void k_mad_above(const float* __restrict__ plan
   const float* x = planes + (size_t)s * plane;
   const uint32_t lo = st[s].prefix;
   uint32_t best = 0xffffffffu;
-  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < plane; i += (size_t)gridDim.x * kT) {
-    if (MASKED) {
-      const int r = (int)(i / w), c = (int)(i % w);
-      if (!in_noise_mask(r, c, inc, exc)) continue;
-    }
-    const uint32_t key = __float_as_uint(x[i]) & 0x7fffffffu;
+  auto take = [&](float v) {
+    const uint32_t key = __float_as_uint(v) & 0x7fffffffu;
     if (key > lo && key < best) best = key;
+  };
+  const size_t stride = (size_t)gridDim.x * kT;
+  if (!MASKED && (reinterpret_cast<uintptr_t>(x) & 15) == 0) {
+    const float4* x4 = reinterpret_cast<const float4*>(x);
+    const size_t n4 = plane >> 2;
+    for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n4; i += stride) {
+      const float4 a = x4[i];
+      take(a.x), take(a.y), take(a.z), take(a.w);
+    }
+    for (size_t j = 4 * n4 + blockIdx.x * (size_t)kT + threadIdx.x; j < plane; j += stride) take(x[j]);
+  } else {
+    for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < plane; i += stride) {
+      if (MASKED) {
+        const int r = (int)(i / w), c = (int)(i % w);
+        if (!in_noise_mask(r, c, inc, exc)) continue;
+      }
+      take(x[i]);
+    }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
