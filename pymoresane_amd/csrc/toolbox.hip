@@ -334,8 +334,19 @@ __global__ __launch_bounds__(kT) void k_plane_max(const float* __restrict__ in, 
   const int s = blockIdx.y;
   const float* x = in + (size_t)s * plane;
   float m = -INFINITY;
-  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < plane; i += (size_t)gridDim.x * kT)
-    m = fmaxf(m, use_abs ? fabsf(x[i]) : x[i]);
+  auto take = [&](float v) { m = fmaxf(m, use_abs ? fabsf(v) : v); };
+  const size_t stride = (size_t)gridDim.x * kT;
+  size_t tail = 0;
+  if ((reinterpret_cast<uintptr_t>(x) & 15) == 0) {  // float4 loads; the maximum does not depend on order
+    const float4* x4 = reinterpret_cast<const float4*>(x);
+    const size_t n4 = plane >> 2;
+    for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n4; i += stride) {
+      const float4 a = x4[i];
+      take(a.x), take(a.y), take(a.z), take(a.w);
+    }
+    tail = 4 * n4;
+  }
+  for (size_t i = tail + blockIdx.x * (size_t)kT + threadIdx.x; i < plane; i += stride) take(x[i]);
   m = block_max_f32(m);
   if (threadIdx.x == 0) pmax[(size_t)s * gridDim.x + blockIdx.x] = m;
 }
@@ -454,24 +465,34 @@ __global__ __launch_bounds__(kT) void k_ccl_tile(const float* __restrict__ w, in
 }
 
 // unions across tile borders (first row, first and last column of each tile)
+// Unions across tile borders.  Only border pixels take part -- every pixel of a row that starts a
+// tile row (r % T == 0, r > 0), and on the other rows the first and last column of each tile -- so
+// one block walks one row at a time and its threads take that row's border pixels, with 32-bit
+// indices (planes hold fewer than 2^29 pixels).  Unions commute: the forest's roots, each
+// component's smallest index, do not depend on their order.
 __global__ __launch_bounds__(kT) void k_ccl_border(int* __restrict__ lab, int h, int wd, const int* pred) {
   if (pred_off(pred)) return;
   constexpr int T = kCclTile;
   const int s = blockIdx.y;
-  const size_t plane = (size_t)h * wd;
-  int* l = lab + (size_t)s * plane;
-  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < plane; i += (size_t)gridDim.x * kT) {
-    const int r = (int)(i / wd), c = (int)(i % wd);
-    const int cr = c % T;
-    const bool top = r > 0 && r % T == 0, left = c > 0 && cr == 0, right = cr == T - 1;
-    if (!(top || left || right) || l[i] < 0) continue;
-    const int me = (int)i;
-    if (left && l[i - 1] >= 0) ccl_union(l, me, me - 1);
-    if (r > 0) {
-      const int up = me - wd;
-      if (c > 0 && (top || left) && l[up - 1] >= 0) ccl_union(l, me, up - 1);
-      if (top && l[up] >= 0) ccl_union(l, me, up);
-      if (c + 1 < wd && (top || right) && l[up + 1] >= 0) ccl_union(l, me, up + 1);
+  int* l = lab + (size_t)s * h * wd;
+  const int edge_cols = 2 * ((wd + T - 1) / T);  // columns tT and tT + T - 1 of every tile t
+  for (int r = blockIdx.x; r < h; r += gridDim.x) {
+    const bool top = r > 0 && r % T == 0;
+    const int n = top ? wd : edge_cols;
+    for (int k = threadIdx.x; k < n; k += kT) {
+      const int c = top ? k : (k >> 1) * T + ((k & 1) ? T - 1 : 0);
+      if (c >= wd) continue;
+      const int cr = c % T;
+      const bool left = c > 0 && cr == 0, right = cr == T - 1;
+      const int me = r * wd + c;
+      if (!(top || left || right) || l[me] < 0) continue;
+      if (left && l[me - 1] >= 0) ccl_union(l, me, me - 1);
+      if (r > 0) {
+        const int up = me - wd;
+        if (c > 0 && (top || left) && l[up - 1] >= 0) ccl_union(l, me, up - 1);
+        if (top && l[up] >= 0) ccl_union(l, me, up);
+        if (c + 1 < wd && (top || right) && l[up + 1] >= 0) ccl_union(l, me, up + 1);
+      }
     }
   }
 }
@@ -480,7 +501,22 @@ __global__ __launch_bounds__(kT) void k_ccl_compress(int* __restrict__ lab, size
   if (pred_off(pred)) return;
   const int s = blockIdx.y;
   int* l = lab + (size_t)s * plane;
-  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < plane; i += (size_t)gridDim.x * kT)
+  const size_t stride = (size_t)gridDim.x * kT;
+  size_t tail = 0;
+  if ((reinterpret_cast<uintptr_t>(l) & 15) == 0) {  // four labels per load; roots found per element
+    const int4* l4 = reinterpret_cast<const int4*>(l);
+    const size_t n4 = plane >> 2;
+    for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n4; i += stride) {
+      const int4 v = l4[i];
+      const int b = (int)(4 * i);
+      if (v.x >= 0) l[b] = ccl_find(l, b);
+      if (v.y >= 0) l[b + 1] = ccl_find(l, b + 1);
+      if (v.z >= 0) l[b + 2] = ccl_find(l, b + 2);
+      if (v.w >= 0) l[b + 3] = ccl_find(l, b + 3);
+    }
+    tail = 4 * n4;
+  }
+  for (size_t i = tail + blockIdx.x * (size_t)kT + threadIdx.x; i < plane; i += stride)
     if (l[i] >= 0) l[i] = ccl_find(l, (int)i);
 }
 
@@ -860,7 +896,7 @@ void source_extract(const float* w, int S, int h, int wd, double tol, int neg_co
   const int tiles = ceil_div(h, kCclTile) * ceil_div(wd, kCclTile);
   hipLaunchKernelGGL(k_ccl_tile, dim3(tiles, S), dim3(kT), 0, st, w, lab, h, wd, pred);
   PMS_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_ccl_border, dim3(gl, S), dim3(kT), 0, st, lab, h, wd, pred);
+  hipLaunchKernelGGL(k_ccl_border, dim3(std::min(h, 4096), S), dim3(kT), 0, st, lab, h, wd, pred);
   PMS_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_ccl_compress, dim3(gl, S), dim3(kT), 0, st, lab, plane, pred);
   PMS_LAUNCH_CHECK();
