@@ -9,10 +9,14 @@ Semantics restated:
   applied separably, axis 0 first and then axis 1 (iuwt.py:163-189), with its
   taps dilated to ``0, +-2**s, +-2**(s+1)`` at scale ``s``.
 * Out-of-range taps use the half-sample symmetric extension
-  ``... c b a | a b c ... x y z | z y x ...`` (iuwt.py:166, 169, 172, 175).
-  The restatement requires ``2**(s+1) <= side`` on both axes, the range in
-  which the reference's slice arithmetic is exactly that extension; beyond it
-  the reference degrades to numpy slice clamping, which is not reproduced.
+  ``... c b a | a b c ... x y z | z y x ...`` (iuwt.py:166, 169, 172, 175)
+  for every tap offset ``o`` with ``|o| <= side``: there the reference's
+  slices reflect once.  A tap offset longer than the axis (``2**s`` or
+  ``2**(s+1)`` above the side, which happens when a scale count chosen for the
+  whole image is applied to a smaller subregion) makes numpy clip both slices
+  of that tap to the whole axis, read in reverse: every output ``i`` then
+  takes input ``side - 1 - i`` for that tap.  The restatement reproduces that
+  too (``tap_index``).
 * Decomposition (iuwt.py:94-106): ``scale_adjust`` pre-smoothings, then per
   scale ``C = h_i*C0; C1 = h_i*C; w_i = C0 - C1; C0 = C``.
 * Recomposition (iuwt.py:131-147): ``rec = smoothed or 0``, then
@@ -35,11 +39,29 @@ def reflect_index(idx, n):
     return np.where(m >= n, 2 * n - 1 - m, m)
 
 
+def tap_index(idx, off, n):
+    """Input index of the tap at offset ``off`` for outputs ``idx`` on an axis of length ``n``.
+
+    ``|off| <= n``: the half-sample symmetric extension (one reflection suffices).
+    ``|off| > n``: the whole axis reversed, ``n - 1 - idx`` (numpy's clipped slices at
+    iuwt.py:166-187 when ``2**s`` or ``2**(s+1)`` exceeds the side).
+    """
+    if abs(off) > n:
+        return n - 1 - idx
+    return reflect_index(idx + off, n)
+
+
 def check_scale(shape, scale):
+    """Raise unless every tap of scale ``scale`` reflects once (the strip-parallel restatement's range)."""
     d2 = 2 << scale
     if d2 > min(shape[-2], shape[-1]):
         raise ValueError(
             "scale %d needs 2**(scale+1)=%d <= image side %s" % (scale, d2, tuple(shape[-2:])))
+
+
+def in_reflect_range(shape, scale):
+    """True when every tap of scale ``scale`` is within one side (``2**(scale+1) <= side``)."""
+    return (2 << scale) <= min(shape[-2], shape[-1])
 
 
 _SCIPY_BACKEND = False
@@ -68,8 +90,9 @@ def _a_trous_scipy(c0, scale):
 def a_trous(c0, scale):
     """One B3 a trous smoothing at dilation 2**scale (iuwt.py:151-191)."""
     c0 = np.asarray(c0, dtype=np.float64)
-    check_scale(c0.shape, scale)
-    if _SCIPY_BACKEND:
+    if scale < 0:
+        raise ValueError("negative scale")
+    if _SCIPY_BACKEND and in_reflect_range(c0.shape, scale):
         return _a_trous_scipy(c0, scale)
     d = 1 << scale
     h, w = c0.shape
@@ -77,11 +100,11 @@ def a_trous(c0, scale):
     rows = np.arange(h)
     vert = np.zeros((h, w))
     for off, coeff in taps:
-        vert += coeff * c0[reflect_index(rows + off, h), :]
+        vert += coeff * c0[tap_index(rows, off, h), :]
     cols = np.arange(w)
     out = np.zeros((h, w))
     for off, coeff in taps:
-        out += coeff * vert[:, reflect_index(cols + off, w)]
+        out += coeff * vert[:, tap_index(cols, off, w)]
     return out
 
 

@@ -215,6 +215,11 @@ class OracleImage(object):
                     break
                 if min_scale == 0:
                     break
+                if exit_code == EXIT_CAP:
+                    # main.py:274-462 would repeat this minor loop unchanged forever: min_scale > 0
+                    # skips the decomposition and nothing else moves
+                    raise RuntimeError("minor loop hit its cap at SNR <= 10.5 with min_scale > 0: "
+                                       "the reference loops forever here")
 
             if min_scale == scale_count:
                 trace.append(("poor", major))

@@ -3,9 +3,10 @@
 Follows pymoresane/iuwt_toolbox.py:17-154 and 292-306.
 
 * ``estimate_threshold`` (17-48): per scale ``median(|w[mask]|) / 0.6745``;
-  the optional mask drops an ``edge_excl`` border and a centred square of
-  half-width ``int_excl`` around ``mid = shape[1] // 2`` (both axes use the
-  same ``mid``, 29 and 40).
+  the optional mask drops an ``edge_excl`` border and a square of
+  half-width ``int_excl`` around ``mid = in1.shape[1] // 2`` -- the row count
+  of the (scales, rows, cols) stack, used for both axes (29 and 40), so on a
+  non-square plane the square sits on the diagonal, not at the centre.
 * ``apply_threshold`` (50-75): keeps ``w`` where ``|w| > sigma_level * t``
   (strict), negatives included.
 * ``source_extraction`` (97-154): 8-connected labels of the non-zero pixels
@@ -26,7 +27,7 @@ EIGHT_CONNECTED = np.ones((3, 3), dtype=int)
 def noise_mask(shape, edge_excl=0, int_excl=0):
     """Boolean mask of the pixels that enter the MAD estimate (iuwt_toolbox.py:31-43)."""
     h, w = shape
-    mid = w // 2
+    mid = h // 2  # in1.shape[1] of the (scales, rows, cols) stack, for both axes
     mask = np.ones((h, w), dtype=bool)
     if edge_excl != 0:
         mask[:] = False

@@ -52,9 +52,35 @@ def test_a_trous_brute_force_direct_sum():
         np.testing.assert_allclose(iuwt.a_trous(x, s), out, atol=1e-14)
 
 
-def test_a_trous_rejects_scale_beyond_side():
-    with pytest.raises(ValueError):
-        iuwt.a_trous(np.zeros((8, 8)), 3)
+def slice_a_trous(c0, scale):
+    """The reference's slice arithmetic for one smoothing (iuwt.py:163-189): per tap an in-range
+    shifted slice plus a reversed border slice, with numpy clipping both when the tap offset
+    exceeds the axis; axis 0 first, then axis 1 (as axis 0 of the transpose)."""
+    def axis0(a):
+        out = iuwt.B3[2] * a
+        for k, off in ((0, 2 << scale), (1, 1 << scale)):
+            out[off:] += iuwt.B3[k] * a[:-off]
+            out[:off] += iuwt.B3[k] * a[off - 1::-1]
+        for k, off in ((3, 1 << scale), (4, 2 << scale)):
+            out[:-off] += iuwt.B3[k] * a[off:]
+            out[-off:] += iuwt.B3[k] * a[:-off - 1:-1]
+        return out
+    return axis0(axis0(np.asarray(c0, dtype=np.float64)).T.copy()).T
+
+
+@pytest.mark.parametrize("shape", [(8, 8), (16, 12), (6, 10), (32, 32)])
+def test_a_trous_equals_reference_slices_at_every_scale(shape):
+    """Including scales whose taps are longer than the side (2**(s+1) > side), where numpy's
+    clipped slices read the whole axis reversed."""
+    x = RNG.standard_normal(shape)
+    for s in range(int(np.log2(max(shape))) + 2):
+        np.testing.assert_allclose(iuwt.a_trous(x, s), slice_a_trous(x, s), rtol=0, atol=1e-14)
+
+
+def test_over_long_scales_keep_perfect_reconstruction():
+    x = RNG.standard_normal((16, 16))
+    det, smooth = iuwt.iuwt_decomposition(x, 6, store_smoothed=True)  # scales 3..5 exceed the side
+    np.testing.assert_allclose(iuwt.iuwt_recomposition(det, 0, smooth), x, atol=1e-12)
 
 
 @pytest.mark.parametrize("adjust", [0, 1, 2])
@@ -167,6 +193,17 @@ def test_estimate_threshold_masks():
     keep[4:6, 4:6] = False
     vals = np.sort(x[0][keep])
     assert tools.estimate_threshold(x, int_excl=1)[0] == pytest.approx(vals[47:49].mean() / 0.6745)
+
+
+def test_estimate_threshold_int_excl_uses_the_row_count_for_both_axes():
+    """iuwt_toolbox.py:29,40: mid = in1.shape[1]/2 -- the row count of the (scales, rows, cols)
+    stack -- bounds the excluded square on both axes, so a 6 x 14 plane with int_excl=2 drops rows
+    and columns 1..4 (mid 3), not columns 5..8 around the column centre."""
+    x = np.arange(1.0, 85.0).reshape(1, 6, 14)
+    keep = np.ones((6, 14), bool)
+    keep[1:5, 1:5] = False
+    vals = np.sort(x[0][keep])  # 68 values: the median is the mean of ranks 33 and 34
+    assert tools.estimate_threshold(x, int_excl=2)[0] == pytest.approx(vals[33:35].mean() / 0.6745, rel=1e-15)
 
 
 def test_apply_threshold_is_strict_and_keeps_negatives():
