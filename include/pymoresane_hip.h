@@ -144,7 +144,11 @@ typedef struct pms_trace_event {
 } pms_trace_event;
 
 /* Arguments of DataImage.moresane (pymoresane/main.py:53-57); 0 for
- * subregion/scale_count means "None" there. conv_mode: 0 linear, 1 circular. */
+ * subregion/scale_count means "None" there. conv_mode: 0 linear, 1 circular.
+ * precision: 0 = float32 bulk arithmetic (wavelet planes, convolutions, CG
+ * vectors, model and residual) with float64 inner products, step sizes and
+ * norms; 1 = float64 throughout the device work, the arithmetic of the
+ * reference's serial numpy path.  Inputs and outputs stay float32 either way. */
 typedef struct pms_moresane_params {
   int n;      /* dirty image side (square) */
   int psf_n;  /* PSF side: n or 2n */
@@ -157,7 +161,7 @@ typedef struct pms_moresane_params {
   int edge_suppression;
   int edge_offset;
   double flux_threshold;
-  int neg_comp, edge_excl, int_excl, pad;
+  int neg_comp, edge_excl, int_excl, precision;
 } pms_moresane_params;
 
 /* One MORESANE run on device buffers: model_inout accumulates the model

@@ -38,7 +38,7 @@ class MoresaneParams(ctypes.Structure):
                 ("major_loop_miter", c_int), ("minor_loop_miter", c_int), ("conv_mode", c_int),
                 ("enforce_positivity", c_int), ("edge_suppression", c_int), ("edge_offset", c_int),
                 ("flux_threshold", c_double), ("neg_comp", c_int), ("edge_excl", c_int), ("int_excl", c_int),
-                ("pad", c_int)]
+                ("precision", c_int)]
 
 
 class BatchJob(ctypes.Structure):

@@ -19,7 +19,8 @@ from .main import make_params
 
 DEFAULTS = dict(subregion=None, scale_count=None, sigma_level=4, loop_gain=0.1, tolerance=0.75, accuracy=1e-6,
                 major_loop_miter=100, minor_loop_miter=30, conv_mode="linear", enforce_positivity=False,
-                edge_suppression=False, edge_offset=0, flux_threshold=0, neg_comp=False, edge_excl=0, int_excl=0)
+                edge_suppression=False, edge_offset=0, flux_threshold=0, neg_comp=False, edge_excl=0, int_excl=0,
+                precision="float32")
 
 
 def params_for(n, psf_n, **kw):
@@ -31,7 +32,7 @@ def params_for(n, psf_n, **kw):
     return make_params(n, psf_n, k["subregion"], k["scale_count"], k["sigma_level"], k["loop_gain"], k["tolerance"],
                        k["accuracy"], k["major_loop_miter"], k["minor_loop_miter"], k["conv_mode"],
                        k["enforce_positivity"], k["edge_suppression"], k["edge_offset"], k["flux_threshold"],
-                       k["neg_comp"], k["edge_excl"], k["int_excl"])
+                       k["neg_comp"], k["edge_excl"], k["int_excl"], k["precision"])
 
 
 class DeviceJobs(object):
@@ -52,6 +53,9 @@ class DeviceJobs(object):
         for it in self.items:
             it["model"].fill(0.0)
             check(_lib.lib().pms_memcpy_d2d(it["resid"].ptr, it["dirty"].ptr, it["dirty"].nbytes, None))
+        # the fills and copies above run on the null stream, which the runners' non-blocking streams
+        # do not wait for
+        _lib.synchronize()
 
     def results(self):
         return [(it["model"].get(), it["resid"].get()) for it in self.items]

@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpmship.so")
-SOURCES = ["iuwt.hip", "fft.hip", "toolbox.hip", "engine.hip", "capi.hip", "serve.hip"]
+SOURCES = ["iuwt.hip", "iuwt_any.hip", "fft.hip", "fft_any.hip", "toolbox.hip", "engine.hip", "capi.hip", "serve.hip"]
 ARCH = os.environ.get("PMS_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -423,6 +423,9 @@ int pms_batch_run(pms_batch_job* jobs, int n_jobs, int n_workers, int max_batch)
     if (n_workers > (int)classes.size()) n_workers = (int)classes.size();
     int dev = 0;
     PMS_HIP(hipGetDevice(&dev));
+    // the runners' streams are non-blocking: work the caller queued on other streams (the null stream
+    // included) for these buffers must be complete before the first job step reads them
+    PMS_HIP(hipDeviceSynchronize());
     std::atomic<int> next_class{0};
     const auto t0 = std::chrono::steady_clock::now();
     auto since = [&](std::chrono::steady_clock::time_point t) {

@@ -46,6 +46,21 @@ struct Geometry {
   int n, psf_n, sub, S, lo;
   bool linear, double_psf;
   int p_sub, p_full;
+  bool f64;  // float64 bulk arithmetic (the precision mode) instead of float32
+};
+
+// Per-element-type pieces of the driver: spectrum element, FFT convolution job.
+template <class T>
+struct EngineTypes;
+template <>
+struct EngineTypes<float> {
+  using C = float2;
+  using Conv = FftConvJob;
+};
+template <>
+struct EngineTypes<double> {
+  using C = Cx<double>;
+  using Conv = GConvJob<double>;
 };
 
 Geometry make_geometry(int n, int psf_n, const MoresaneParams& prm, int scale_count_in, int* subregion_out);
@@ -53,6 +68,7 @@ Geometry make_geometry(int n, int psf_n, const MoresaneParams& prm, int scale_co
 // byte sizes of the per-job device buffers
 struct JobSizes {
   size_t s2, S_s2, mask, iuwt, N2, spec_sub, spec_full, fft, psf, ext, partial, pmax;
+  bool f64;
 };
 
 // device bytes one job of this geometry needs (its workspace plus nothing else)
@@ -62,7 +78,22 @@ size_t job_footprint_bytes(const Geometry& g);
 constexpr int kSlotDoubles = 64;
 constexpr int kSlotInts = 64;
 
-// Device buffers of one deconvolution, kept across calls of the same or smaller shape.
+// The element-typed device buffers of one job (sub = subregion side, S = scale capacity).
+template <class T>
+struct Planes {
+  using C = typename EngineTypes<T>::C;
+  T *dirty_sub, *psf_sub, *D, *Dcg, *E;
+  T* Rs;  // E - M.W(H x) of the running minor loop (incremental SNR)
+  T *rec, *x, *xn, *r, *rn, *p, *ap, *cbuf;
+  T* iuwt_work;
+  T *model_local, *resid, *conv_full, *psf_frame;
+  T* pmax;
+  T* cur_dirty;  // moresane_by_scale: the dirty image of the current call (a private copy)
+  C *spec_sub, *spec_full, *spec_full_buf, *fft_work;
+};
+
+// Device buffers of one deconvolution, kept across calls of the same or smaller shape and the same
+// precision.  Only the planes of the job's element type are allocated (the other set stays null).
 class Job {
  public:
   Job() = default;
@@ -71,33 +102,38 @@ class Job {
   Job& operator=(const Job&) = delete;
   // grow-only allocation for geometry g; returns true when buffers moved
   bool ensure(const Geometry& g, hipStream_t st);
-  float* by_scale_dirty(size_t n2);  // private copy of the dirty image for moresane_by_scale
+  template <class T>
+  T* by_scale_dirty(size_t n2);  // private copy of the dirty image for moresane_by_scale
   size_t device_bytes() const { return arena_.bytes() + by_scale_arena_.bytes(); }
   const Geometry& geometry() const { return geo_; }
+  template <class T>
+  Planes<T>& pl();
 
  private:
   DeviceArena arena_;
   DeviceArena by_scale_arena_;
-  size_t by_scale_n2_ = 0;
+  size_t by_scale_bytes_ = 0;
+  void* by_scale_ptr_ = nullptr;
   Geometry geo_{};
   JobSizes cap_{};
   bool have_ = false;
 
  public:
-  // device buffers (sub = subregion side, S = scale capacity)
-  float *dirty_sub = nullptr, *psf_sub = nullptr, *D = nullptr, *Dcg = nullptr, *E = nullptr;
-  float* Rs = nullptr;  // E - M.W(H x) of the running minor loop (incremental SNR)
+  Planes<float> f{};
+  Planes<double> d{};
   uint8_t* M = nullptr;
-  float *rec = nullptr, *x = nullptr, *xn = nullptr, *r = nullptr, *rn = nullptr, *p = nullptr, *ap = nullptr,
-        *cbuf = nullptr;
-  float* iuwt_work = nullptr;
-  float *model_local = nullptr, *resid = nullptr, *conv_full = nullptr, *psf_frame = nullptr;
-  float2 *spec_sub = nullptr, *spec_full = nullptr, *spec_full_buf = nullptr, *fft_work = nullptr;
   void *mad_work = nullptr, *ext_work = nullptr;
   double* partial = nullptr;
-  float* pmax = nullptr;
-  float* cur_dirty = nullptr;
 };
+
+template <>
+inline Planes<float>& Job::pl<float>() {
+  return f;
+}
+template <>
+inline Planes<double>& Job::pl<double>() {
+  return d;
+}
 
 // One request: moresane (main.py:53-521) or moresane_by_scale (main.py:523-599) of one image.
 struct TaskSpec {
@@ -113,21 +149,38 @@ struct TaskSpec {
 class Runner;
 
 // Work of one step that the runner issues as batched launches across tasks (between every task's
-// pre and post work of that step), keyed by plane side or (image side, FFT frame).
-struct SharedOps {
-  std::vector<std::pair<int, IuwtDecompJob>> decomp;
-  std::vector<std::pair<int, IuwtRecompJob>> recomp;
-  std::vector<std::pair<std::pair<int, int>, FftConvJob>> conv;
+// pre and post work of that step), keyed by plane side or (image side, FFT frame), per element type.
+template <class T>
+struct SharedOpsT {
+  std::vector<std::pair<int, IuwtDecompJobT<T>>> decomp;
+  std::vector<std::pair<int, IuwtRecompJobT<T>>> recomp;
+  std::vector<std::pair<std::pair<int, int>, typename EngineTypes<T>::Conv>> conv;
 };
+struct SharedOps {
+  SharedOpsT<float> f;
+  SharedOpsT<double> d;
+  template <class T>
+  SharedOpsT<T>& of();
+};
+template <>
+inline SharedOpsT<float>& SharedOps::of<float>() {
+  return f;
+}
+template <>
+inline SharedOpsT<double>& SharedOps::of<double>() {
+  return d;
+}
 
 // Buffers and scales of one CG iteration of one job (also the key of a captured iteration graph).
+// The vector pointers are of the job's element type (f64 selects double).
 struct IterArgs {
-  float *x, *xn, *r, *rn, *p, *ap;
+  void *x, *xn, *r, *rn, *p, *ap;
   double *rr, *rnrn;
-  int max_scale, adj, enforce, sub, p_sub;
+  int max_scale, adj, enforce, sub, p_sub, f64;
   bool operator<(const IterArgs& o) const {
-    return std::tie(x, xn, r, rn, p, ap, rr, rnrn, max_scale, adj, enforce, sub, p_sub) <
-           std::tie(o.x, o.xn, o.r, o.rn, o.p, o.ap, o.rr, o.rnrn, o.max_scale, o.adj, o.enforce, o.sub, o.p_sub);
+    return std::tie(x, xn, r, rn, p, ap, rr, rnrn, max_scale, adj, enforce, sub, p_sub, f64) <
+           std::tie(o.x, o.xn, o.r, o.rn, o.p, o.ap, o.rr, o.rnrn, o.max_scale, o.adj, o.enforce, o.sub, o.p_sub,
+                    o.f64);
   }
 };
 
@@ -170,9 +223,18 @@ class Task {
   void choose_scales();
   void minor_exit();
   void after_inner();
+  template <class T>
+  void enqueue_pre_t(hipStream_t st);
+  template <class T>
+  void add_shared_t(SharedOps& ops);
+  template <class T>
+  void enqueue_post_t(hipStream_t st);
+  template <class T>
   void enqueue_setup(hipStream_t st);
+  template <class T>
   void enqueue_finish(hipStream_t st);
-  const float* dirty_dev() const { return spec.by_scale ? job_->cur_dirty : spec.dirty; }
+  bool f64() const { return spec.prm.precision != 0; }
+  double read_dmax(const double* slot_host) const;
 
   Job* job_;
   double* scal_;          // device slot (kSlotDoubles doubles)
@@ -190,13 +252,13 @@ class Task {
   int upload_energies_ = 0;  // energies waiting in upload_ for the next enqueue
   int major_ = 0, min_scale_ = 0, max_scale_ = 0, scale_adjust_ = 0;
   double max_coeff_ = 1.0, std_current_ = 1000.0, std_ratio_ = 1.0;
-  float dmax_ = 0.f;
+  double dmax_ = 0.0;
   std::vector<double> nsm_;
   // minor loop
   bool minor_first_ = false, iterate_ = false;
   int niter_ = 0, exit_code_ = EXIT_CAP;
   double e2_ = 0.0, snr_last_ = 0.0, snr_current_ = 0.0;
-  float *x_ = nullptr, *xn_ = nullptr, *r_ = nullptr, *rn_ = nullptr;
+  void *x_ = nullptr, *xn_ = nullptr, *r_ = nullptr, *rn_ = nullptr;  // the job's element type
   double *rr_ = nullptr, *rnrn_ = nullptr;
 };
 
@@ -229,8 +291,13 @@ class Runner {
 
  private:
   void run_shared(const std::vector<int>& active);
+  template <class T>
+  void run_shared_t(SharedOpsT<T>& ops);
   void run_iterations(std::vector<Task*>& ts);
   void enqueue_iterations(const std::vector<IterArgs>& its, const std::vector<Task*>& ts);
+  template <class T>
+  void enqueue_iterations_t(const std::vector<IterArgs>& its, const std::vector<Task*>& ts);
+  template <class T>
   void apply_op(const std::vector<Task*>& ts, const std::vector<IterArgs>& its, bool clip_pass);
   void drop_graphs();
 

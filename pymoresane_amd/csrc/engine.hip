@@ -40,21 +40,33 @@ constexpr int kThr = 0, kNsm = 16, kEnergy = 32, kRR = 48, kPAP = 49, kAlpha = 5
               kD2 = 53, kE2 = 54, kMeanSum = 56, kDev2 = 57, kDmax = 60;
 constexpr int kClip = 0, kBorders = 16;
 
+// 16 bytes of plane elements: four floats or two doubles (the side is even, so a plane holds a
+// multiple of four elements and every job's vector starts 16-byte aligned).
+template <class T>
+struct alignas(16) V16 {
+  T a[16 / sizeof(T)];
+};
+template <class T>
+constexpr int kLanes = 16 / sizeof(T);
+
 // Per-job pointers of one batched CG launch (grid y selects the job).
+template <class T>
 struct CgJob {
-  float *x, *xn, *r, *rn, *p, *ap;
+  T *x, *xn, *r, *rn, *p, *ap;
   double *partial, *scal, *rr, *rnrn;
   int* iflag;
   int enforce;
 };
+template <class T>
 struct CgBatch {
-  CgJob j[kMaxBatch];
+  CgJob<T> j[kMaxBatch];
   size_t n;
 };
 
-// ---- CG vector kernels.  Planes hold a multiple of 4 floats (the side is even), so each thread
-// walks float4 chunks.  Reductions write one partial per block; a one-block-per-job finishing
-// kernel adds the partials in block order (kernel boundaries order the two, no fences needed).
+// ---- CG vector kernels, in the job's element type T (float, or double in the float64 precision
+// mode); every step size and inner product is float64.  Reductions write one partial per block; a
+// one-block-per-job finishing kernel adds the partials in block order (kernel boundaries order the
+// two, no fences needed).
 __device__ __forceinline__ double block_total(const double* __restrict__ partial, int np) {
   double acc = 0.0;
   for (int i = threadIdx.x; i < np; i += kT) acc += partial[i];
@@ -72,37 +84,41 @@ __device__ __forceinline__ double block_total_all(const double* __restrict__ par
   return s_tot;
 }
 
-__device__ __forceinline__ float block_min_all(const double* __restrict__ partial, int np) {
-  __shared__ float s_min;
-  float m = INFINITY;
-  for (int i = threadIdx.x; i < np; i += kT) m = fminf(m, (float)partial[i]);
-  m = block_min_f32(m);
+template <class T>
+__device__ __forceinline__ T block_min_all(const double* __restrict__ partial, int np) {
+  __shared__ T s_min;
+  T m = INFINITY;
+  for (int i = threadIdx.x; i < np; i += kT) {
+    const T v = (T)partial[i];
+    m = v < m ? v : m;
+  }
+  m = block_min_t<T>(m);
   if (threadIdx.x == 0) s_min = m;
   __syncthreads();
   return s_min;
 }
 
 // per-block partials of p . Ap
-__global__ __launch_bounds__(kT) void k_cg_pap(CgBatch b) {
-  const CgJob& J = b.j[blockIdx.y];
-  const float4* __restrict__ p = reinterpret_cast<const float4*>(J.p);
-  const float4* __restrict__ ap = reinterpret_cast<const float4*>(J.ap);
-  const size_t n4 = b.n >> 2;
+template <class T>
+__global__ __launch_bounds__(kT) void k_cg_pap(CgBatch<T> b) {
+  const CgJob<T>& J = b.j[blockIdx.y];
+  const V16<T>* __restrict__ p = reinterpret_cast<const V16<T>*>(J.p);
+  const V16<T>* __restrict__ ap = reinterpret_cast<const V16<T>*>(J.ap);
+  const size_t nv = b.n / kLanes<T>;
   double acc = 0.0;
-  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n4; i += (size_t)gridDim.x * kT) {
-    const float4 u = p[i], v = ap[i];
-    acc = fma((double)u.x, (double)v.x, acc);
-    acc = fma((double)u.y, (double)v.y, acc);
-    acc = fma((double)u.z, (double)v.z, acc);
-    acc = fma((double)u.w, (double)v.w, acc);
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < nv; i += (size_t)gridDim.x * kT) {
+    const V16<T> u = p[i], v = ap[i];
+#pragma unroll
+    for (int k = 0; k < kLanes<T>; ++k) acc = fma((double)u.a[k], (double)v.a[k], acc);
   }
   acc = block_sum_f64(acc);
   if (threadIdx.x == 0) J.partial[blockIdx.x] = acc;
 }
 
 // alpha = rr / (p . Ap)
-__global__ __launch_bounds__(kT) void k_cg_alpha_fin(CgBatch b, int np) {
-  const CgJob& J = b.j[blockIdx.y];
+template <class T>
+__global__ __launch_bounds__(kT) void k_cg_alpha_fin(CgBatch<T> b, int np) {
+  const CgJob<T>& J = b.j[blockIdx.y];
   const double tot = block_total(J.partial, np);
   if (threadIdx.x == 0) {
     J.scal[kPAP] = tot;
@@ -110,84 +126,86 @@ __global__ __launch_bounds__(kT) void k_cg_alpha_fin(CgBatch b, int np) {
   }
 }
 
-__device__ __forceinline__ float xn_of(double a, float p, float x) { return (float)fma(a, (double)p, (double)x); }
+template <class T>
+__device__ __forceinline__ T xn_of(double a, T p, T x) {
+  return (T)fma(a, (double)p, (double)x);
+}
 
 // xn = x + alpha p ; per-block minima of xn
-__global__ __launch_bounds__(kT) void k_cg_xn(CgBatch b) {
-  const CgJob& J = b.j[blockIdx.y];
+template <class T>
+__global__ __launch_bounds__(kT) void k_cg_xn(CgBatch<T> b) {
+  const CgJob<T>& J = b.j[blockIdx.y];
   const double a = J.scal[kAlpha];
-  const float4* __restrict__ x = reinterpret_cast<const float4*>(J.x);
-  const float4* __restrict__ p = reinterpret_cast<const float4*>(J.p);
-  float4* __restrict__ xn = reinterpret_cast<float4*>(J.xn);
-  const size_t n4 = b.n >> 2;
-  float m = INFINITY;
-  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n4; i += (size_t)gridDim.x * kT) {
-    const float4 u = p[i], v = x[i];
-    float4 o;
-    o.x = xn_of(a, u.x, v.x);
-    o.y = xn_of(a, u.y, v.y);
-    o.z = xn_of(a, u.z, v.z);
-    o.w = xn_of(a, u.w, v.w);
+  const V16<T>* __restrict__ x = reinterpret_cast<const V16<T>*>(J.x);
+  const V16<T>* __restrict__ p = reinterpret_cast<const V16<T>*>(J.p);
+  V16<T>* __restrict__ xn = reinterpret_cast<V16<T>*>(J.xn);
+  const size_t nv = b.n / kLanes<T>;
+  T m = INFINITY;
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < nv; i += (size_t)gridDim.x * kT) {
+    const V16<T> u = p[i], v = x[i];
+    V16<T> o;
+#pragma unroll
+    for (int k = 0; k < kLanes<T>; ++k) {
+      o.a[k] = xn_of<T>(a, u.a[k], v.a[k]);
+      m = o.a[k] < m ? o.a[k] : m;
+    }
     xn[i] = o;
-    m = fminf(m, fminf(fminf(o.x, o.y), fminf(o.z, o.w)));
   }
-  m = block_min_f32(m);
+  m = block_min_t<T>(m);
   if (threadIdx.x == 0) J.partial[blockIdx.x] = (double)m;
 }
 
 // positivity clip when enforce && min(xn) < 0 (the minimum from k_cg_xn's np per-block minima;
 // block 0 stores the flag, which predicates the clip pass's operator launches):
 // xn[xn<0] = 0 ; p = (xn - x) / alpha
-__global__ __launch_bounds__(kT) void k_cg_clip(CgBatch b, int np) {
-  const CgJob& J = b.j[blockIdx.y];
-  const int flag = (J.enforce && block_min_all(J.partial, np) < 0.f) ? 1 : 0;
+template <class T>
+__global__ __launch_bounds__(kT) void k_cg_clip(CgBatch<T> b, int np) {
+  const CgJob<T>& J = b.j[blockIdx.y];
+  const int flag = (J.enforce && block_min_all<T>(J.partial, np) < (T)0) ? 1 : 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) J.iflag[kClip] = flag;
   if (flag == 0) return;
   const double a = J.scal[kAlpha];
-  float4* __restrict__ xn = reinterpret_cast<float4*>(J.xn);
-  const float4* __restrict__ x = reinterpret_cast<const float4*>(J.x);
-  float4* __restrict__ p = reinterpret_cast<float4*>(J.p);
-  const size_t n4 = b.n >> 2;
-  auto clip = [a](float& v, float xv) {
-    if (v < 0.f) v = 0.f;
-    return (float)(((double)v - (double)xv) / a);
-  };
-  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n4; i += (size_t)gridDim.x * kT) {
-    float4 v = xn[i];
-    const float4 u = x[i];
-    float4 q;
-    q.x = clip(v.x, u.x);
-    q.y = clip(v.y, u.y);
-    q.z = clip(v.z, u.z);
-    q.w = clip(v.w, u.w);
+  V16<T>* __restrict__ xn = reinterpret_cast<V16<T>*>(J.xn);
+  const V16<T>* __restrict__ x = reinterpret_cast<const V16<T>*>(J.x);
+  V16<T>* __restrict__ p = reinterpret_cast<V16<T>*>(J.p);
+  const size_t nv = b.n / kLanes<T>;
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < nv; i += (size_t)gridDim.x * kT) {
+    V16<T> v = xn[i];
+    const V16<T> u = x[i];
+    V16<T> q;
+#pragma unroll
+    for (int k = 0; k < kLanes<T>; ++k) {
+      if (v.a[k] < (T)0) v.a[k] = 0;
+      q.a[k] = (T)(((double)v.a[k] - (double)u.a[k]) / a);
+    }
     xn[i] = v;
     p[i] = q;
   }
 }
 
-__device__ __forceinline__ float rn_of(double a, float ap, float r) { return (float)fma(-a, (double)ap, (double)r); }
+template <class T>
+__device__ __forceinline__ T rn_of(double a, T ap, T r) {
+  return (T)fma(-a, (double)ap, (double)r);
+}
 
 // rn = r - alpha Ap ; per-block partials of rn . rn
-__global__ __launch_bounds__(kT) void k_cg_rn(CgBatch b) {
-  const CgJob& J = b.j[blockIdx.y];
+template <class T>
+__global__ __launch_bounds__(kT) void k_cg_rn(CgBatch<T> b) {
+  const CgJob<T>& J = b.j[blockIdx.y];
   const double a = J.scal[kAlpha];
-  const float4* __restrict__ r = reinterpret_cast<const float4*>(J.r);
-  const float4* __restrict__ ap = reinterpret_cast<const float4*>(J.ap);
-  float4* __restrict__ rn = reinterpret_cast<float4*>(J.rn);
-  const size_t n4 = b.n >> 2;
+  const V16<T>* __restrict__ r = reinterpret_cast<const V16<T>*>(J.r);
+  const V16<T>* __restrict__ ap = reinterpret_cast<const V16<T>*>(J.ap);
+  V16<T>* __restrict__ rn = reinterpret_cast<V16<T>*>(J.rn);
+  const size_t nv = b.n / kLanes<T>;
   double acc = 0.0;
-  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n4; i += (size_t)gridDim.x * kT) {
-    const float4 u = ap[i], v = r[i];
-    float4 o;
-    o.x = rn_of(a, u.x, v.x);
-    o.y = rn_of(a, u.y, v.y);
-    o.z = rn_of(a, u.z, v.z);
-    o.w = rn_of(a, u.w, v.w);
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < nv; i += (size_t)gridDim.x * kT) {
+    const V16<T> u = ap[i], v = r[i];
+    V16<T> o;
+#pragma unroll
+    for (int k = 0; k < kLanes<T>; ++k) o.a[k] = rn_of<T>(a, u.a[k], v.a[k]);
     rn[i] = o;
-    acc = fma((double)o.x, (double)o.x, acc);
-    acc = fma((double)o.y, (double)o.y, acc);
-    acc = fma((double)o.z, (double)o.z, acc);
-    acc = fma((double)o.w, (double)o.w, acc);
+#pragma unroll
+    for (int k = 0; k < kLanes<T>; ++k) acc = fma((double)o.a[k], (double)o.a[k], acc);
   }
   acc = block_sum_f64(acc);
   if (threadIdx.x == 0) J.partial[blockIdx.x] = acc;
@@ -195,24 +213,23 @@ __global__ __launch_bounds__(kT) void k_cg_rn(CgBatch b) {
 
 // rnrn = rn . rn (from k_cg_rn's np partials) ; beta = rnrn / rr (block 0 stores both) ;
 // p = rn + beta p
-__global__ __launch_bounds__(kT) void k_cg_p(CgBatch b, int np) {
-  const CgJob& J = b.j[blockIdx.y];
+template <class T>
+__global__ __launch_bounds__(kT) void k_cg_p(CgBatch<T> b, int np) {
+  const CgJob<T>& J = b.j[blockIdx.y];
   const double tot = block_total_all(J.partial, np);
   const double be = tot / *J.rr;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     *J.rnrn = tot;
     J.scal[kBeta] = be;
   }
-  const float4* __restrict__ rn = reinterpret_cast<const float4*>(J.rn);
-  float4* __restrict__ p = reinterpret_cast<float4*>(J.p);
-  const size_t n4 = b.n >> 2;
-  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n4; i += (size_t)gridDim.x * kT) {
-    const float4 u = rn[i];
-    float4 v = p[i];
-    v.x = (float)fma(be, (double)v.x, (double)u.x);
-    v.y = (float)fma(be, (double)v.y, (double)u.y);
-    v.z = (float)fma(be, (double)v.z, (double)u.z);
-    v.w = (float)fma(be, (double)v.w, (double)u.w);
+  const V16<T>* __restrict__ rn = reinterpret_cast<const V16<T>*>(J.rn);
+  V16<T>* __restrict__ p = reinterpret_cast<V16<T>*>(J.p);
+  const size_t nv = b.n / kLanes<T>;
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < nv; i += (size_t)gridDim.x * kT) {
+    const V16<T> u = rn[i];
+    V16<T> v = p[i];
+#pragma unroll
+    for (int k = 0; k < kLanes<T>; ++k) v.a[k] = (T)fma(be, (double)v.a[k], (double)u.a[k]);
     p[i] = v;
   }
 }
@@ -223,63 +240,72 @@ __global__ __launch_bounds__(kT) void k_cg_p(CgBatch b, int np) {
 // decomposition (Dcg).  The difference planes R = E - M.W(H x) are carried across the iterations of
 // a minor loop (R = E at its start, x = 0): R <- R - alpha Dcg, D2 = sum R^2.  This replaces one
 // convolution and one decomposition per iteration by one streaming pass over R and Dcg.
+template <class T>
 struct SnrJob {
-  float* rs;
-  const float* dcg;
+  T* rs;
+  const T* dcg;
   const double* scal;  // alpha at kAlpha
   double* partial;
   size_t n;  // elements: scales x plane (a multiple of 4)
   int nblk;  // blocks and partials of this job (a function of n only)
 };
+template <class T>
 struct SnrBatch {
-  SnrJob j[kMaxBatch];
+  SnrJob<T> j[kMaxBatch];
 };
 
-__global__ __launch_bounds__(kT) void k_cg_snr(SnrBatch b) {
-  const SnrJob& J = b.j[blockIdx.y];
+template <class T>
+__global__ __launch_bounds__(kT) void k_cg_snr(SnrBatch<T> b) {
+  const SnrJob<T>& J = b.j[blockIdx.y];
   if ((int)blockIdx.x >= J.nblk) return;
   const double a = J.scal[kAlpha];
-  float4* __restrict__ rs = reinterpret_cast<float4*>(J.rs);
-  const float4* __restrict__ dcg = reinterpret_cast<const float4*>(J.dcg);
-  const size_t n4 = J.n >> 2;
+  V16<T>* __restrict__ rs = reinterpret_cast<V16<T>*>(J.rs);
+  const V16<T>* __restrict__ dcg = reinterpret_cast<const V16<T>*>(J.dcg);
+  const size_t nv = J.n / kLanes<T>;
   double acc = 0.0;
-  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n4; i += (size_t)J.nblk * kT) {
-    const float4 d = dcg[i];
-    float4 r = rs[i];
-    r.x = rn_of(a, d.x, r.x);
-    r.y = rn_of(a, d.y, r.y);
-    r.z = rn_of(a, d.z, r.z);
-    r.w = rn_of(a, d.w, r.w);
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < nv; i += (size_t)J.nblk * kT) {
+    const V16<T> d = dcg[i];
+    V16<T> r = rs[i];
+#pragma unroll
+    for (int k = 0; k < kLanes<T>; ++k) r.a[k] = rn_of<T>(a, d.a[k], r.a[k]);
     rs[i] = r;
-    acc = fma((double)r.x, (double)r.x, acc);
-    acc = fma((double)r.y, (double)r.y, acc);
-    acc = fma((double)r.z, (double)r.z, acc);
-    acc = fma((double)r.w, (double)r.w, acc);
+#pragma unroll
+    for (int k = 0; k < kLanes<T>; ++k) acc = fma((double)r.a[k], (double)r.a[k], acc);
   }
   acc = block_sum_f64(acc);
   if (threadIdx.x == 0) J.partial[blockIdx.x] = acc;
 }
 
 // model[r0+r, c0+c] += gain * x[r, c]
-__global__ __launch_bounds__(kT) void k_model_add(float* __restrict__ model, int ld, int r0, int c0, int n,
-                                                  const float* __restrict__ x, double gain) {
+template <class T>
+__global__ __launch_bounds__(kT) void k_model_add(T* __restrict__ model, int ld, int r0, int c0, int n,
+                                                  const T* __restrict__ x, double gain) {
   const size_t tot = (size_t)n * n;
   for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < tot; i += (size_t)gridDim.x * kT) {
     const int r = (int)(i / n), c = (int)(i % n);
-    float* m = model + (size_t)(r0 + r) * ld + (c0 + c);
-    *m = *m + (float)(gain * (double)x[i]);
+    T* m = model + (size_t)(r0 + r) * ld + (c0 + c);
+    *m = *m + (T)(gain * (double)x[i]);
   }
 }
 
 // out = a - b
-__global__ __launch_bounds__(kT) void k_sub(const float* __restrict__ a, const float* __restrict__ b,
-                                            float* __restrict__ out, size_t n) {
-  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) out[i] = a[i] - b[i];
+template <class TA, class T>
+__global__ __launch_bounds__(kT) void k_sub(const TA* __restrict__ a, const T* __restrict__ b, T* __restrict__ out,
+                                            size_t n) {
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT)
+    out[i] = (T)a[i] - b[i];
 }
 
-// plane += other
-__global__ __launch_bounds__(kT) void k_add_inplace(float* __restrict__ a, const float* __restrict__ b, size_t n) {
-  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) a[i] += b[i];
+// plane += other (rounded once to the plane's type)
+template <class TA, class TB>
+__global__ __launch_bounds__(kT) void k_add_inplace(TA* __restrict__ a, const TB* __restrict__ b, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT)
+    a[i] = (TA)((TB)a[i] + b[i]);
+}
+
+// float32 copy of a double plane (the float64 mode's residual output)
+__global__ __launch_bounds__(kT) void k_to_f32(const double* __restrict__ a, float* __restrict__ out, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) out[i] = (float)a[i];
 }
 
 int grid_for(size_t n) { return reduce_grid(n); }
@@ -301,23 +327,56 @@ int cg_grid(size_t n) {
   return (int)(b < 1 ? 1 : (b > 1024 ? 1024 : b));
 }
 
-template <class K>
-void launch_cg(K kernel, const CgBatch& b, int nb, hipStream_t st) {
+template <class T, class K>
+void launch_cg(K kernel, const CgBatch<T>& b, int nb, hipStream_t st) {
   hipLaunchKernelGGL(kernel, dim3(cg_grid(b.n), nb), dim3(kT), 0, st, b);
   PMS_LAUNCH_CHECK();
 }
 
-template <class K>
-void launch_cg_fin(K kernel, const CgBatch& b, int nb, hipStream_t st) {
+template <class T, class K>
+void launch_cg_fin(K kernel, const CgBatch<T>& b, int nb, hipStream_t st) {
   hipLaunchKernelGGL(kernel, dim3(1, nb), dim3(kT), 0, st, b, cg_grid(b.n));
   PMS_LAUNCH_CHECK();
 }
 
 // a streaming kernel that also finishes the previous kernel's reduction (cg_grid(n) partials)
-template <class K>
-void launch_cg_after(K kernel, const CgBatch& b, int nb, hipStream_t st) {
+template <class T, class K>
+void launch_cg_after(K kernel, const CgBatch<T>& b, int nb, hipStream_t st) {
   hipLaunchKernelGGL(kernel, dim3(cg_grid(b.n), nb), dim3(kT), 0, st, b, cg_grid(b.n));
   PMS_LAUNCH_CHECK();
+}
+
+// Element-typed front ends of the transform and convolution libraries.
+void decompose_batch(IuwtDecompJobT<float>* jobs, int count, int h, int w, hipStream_t st) {
+  iuwt_decompose_batch(jobs, count, h, w, st);
+}
+void decompose_batch(IuwtDecompJobT<double>* jobs, int count, int h, int w, hipStream_t st) {
+  iuwt_decompose_batch(jobs, count, h, w, st);
+}
+void recompose_batch(const IuwtRecompJobT<float>* jobs, int count, int h, int w, hipStream_t st) {
+  iuwt_recompose_batch(jobs, count, h, w, st);
+}
+void recompose_batch(const IuwtRecompJobT<double>* jobs, int count, int h, int w, hipStream_t st) {
+  iuwt_recompose_batch(jobs, count, h, w, st);
+}
+void conv_batch(const FftConvJob* jobs, int count, int n, int p, hipStream_t st) {
+  fft_conv_batch(jobs, count, n, p, st);
+}
+void conv_batch(const GConvJob<double>* jobs, int count, int n, int p, hipStream_t st) {
+  gfft_conv_batch<double>(jobs, count, n, p, st);
+}
+template <class TIN>
+void psf_spectrum_of(const TIN* psf, int q, int p, float2* spec, float2* work, hipStream_t st) {
+  fft_psf_spectrum(psf, q, p, spec, work, st);
+}
+template <class TIN>
+void psf_spectrum_of(const TIN* psf, int q, int p, Cx<double>* spec, Cx<double>* work, hipStream_t st) {
+  gfft_psf_spectrum<double, TIN>(psf, q, p, spec, work, st);
+}
+template <class T>
+void decompose1(const T* in, int n, int S, T* detail, T* work, hipStream_t st) {
+  IuwtDecompJobT<T> j{in, S, 0, detail, nullptr, work, nullptr, nullptr, nullptr, nullptr, 0};
+  decompose_batch(&j, 1, n, n, st);
 }
 
 }  // namespace
@@ -346,6 +405,7 @@ void* DeviceArena::alloc(size_t bytes) {
 Geometry make_geometry(int n, int psf_n, const MoresaneParams& prm, int scale_count_in, int* subregion_out) {
   Geometry g{};
   PMS_REQUIRE(n >= 4 && (n % 2) == 0, "Image size is uneven. Please use even dimensions.");
+  PMS_REQUIRE(prm.precision == 0 || prm.precision == 1, "precision must be 0 (float32) or 1 (float64)");
   int sub = prm.subregion;
   if (sub <= 0 || sub > n) sub = n;
   PMS_REQUIRE(sub % 2 == 0 && sub >= 4, "subregion must be even and >= 4");
@@ -360,6 +420,7 @@ Geometry make_geometry(int n, int psf_n, const MoresaneParams& prm, int scale_co
   g.lo = n / 2 - sub / 2;
   g.linear = prm.conv_mode == 0;
   g.double_psf = psf_n == 2 * n;
+  g.f64 = prm.precision == 1;
   PMS_REQUIRE(g.double_psf || psf_n == n, "PSF must be the dirty image size or twice it");
   if (g.linear) {
     g.p_sub = 2 * sub;
@@ -368,46 +429,59 @@ Geometry make_geometry(int n, int psf_n, const MoresaneParams& prm, int scale_co
     g.p_sub = sub;
     g.p_full = n;
   }
-  PMS_REQUIRE(is_pow2(g.p_sub) && is_pow2(g.p_full), "FFT convolution needs power-of-two image sides");
+  for (int pf : {g.p_sub, g.p_full}) {
+    if (g.f64)
+      PMS_REQUIRE(gfft_frame_ok(pf, sizeof(double)),
+                  "float64 FFT frame side " + std::to_string(pf) + " is not supported (even sides up to " +
+                      std::to_string(gfft_max_frame(sizeof(double))) + " with prime factors up to 13)");
+    else
+      PMS_REQUIRE(is_pow2(pf) ? pf <= kFftMaxLen : gfft_frame_ok(pf, sizeof(float)),
+                  "FFT frame side " + std::to_string(pf) + " is not supported (even sides up to " +
+                      std::to_string(kFftMaxLen) + " with prime factors up to 13)");
+  }
   check_iuwt_scale(sub, sub, S - 1);
   if (subregion_out) *subregion_out = sub;
   return g;
 }
 
 
-// Byte sizes of every per-job buffer for geometry g (S = scale capacity).
+// Byte sizes of every per-job buffer for geometry g (S = scale capacity), in the job's element type.
 static JobSizes job_sizes(const Geometry& g, int S) {
   JobSizes z{};
+  const size_t e = g.f64 ? sizeof(double) : sizeof(float);
   const size_t N2 = (size_t)g.n * g.n;
   const size_t s2 = (size_t)g.sub * g.sub;
-  z.s2 = s2 * sizeof(float);
-  z.S_s2 = (size_t)S * s2 * sizeof(float);
+  z.f64 = g.f64;
+  z.s2 = s2 * e;
+  z.S_s2 = (size_t)S * s2 * e;
   z.mask = (size_t)S * s2;
-  z.iuwt = 2 * s2 * sizeof(float);
-  z.N2 = N2 * sizeof(float);
-  z.spec_sub = fft_spectrum_elems(g.p_sub) * sizeof(float2);
-  z.spec_full = fft_spectrum_elems(g.p_full) * sizeof(float2);
+  z.iuwt = 2 * s2 * e;
+  z.N2 = N2 * e;
+  z.spec_sub = fft_spectrum_elems(g.p_sub) * 2 * e;
+  z.spec_full = fft_spectrum_elems(g.p_full) * 2 * e;
   z.fft = std::max({fft_conv_work_elems(g.sub, g.p_sub), fft_conv_work_elems(g.n, g.p_full),
                     fft_spectrum_elems(g.p_sub), fft_spectrum_elems(g.p_full)}) *
-          sizeof(float2);
-  z.psf = (size_t)g.psf_n * g.psf_n * sizeof(float);
+          2 * e;
+  z.psf = (size_t)g.psf_n * g.psf_n * e;
   z.ext = extract_work_bytes(S, g.sub, g.sub);
-  z.partial = (size_t)std::max<long>({(long)iuwt_decompose_partials(g.sub, g.sub, S, 0),
-                                      (long)reduce_grid(N2) * 2, (long)reduce_grid(s2) * S, 4096L}) *
+  const long dec_partials = g.f64 ? iuwt_decompose_partials_f64(g.sub, g.sub, S, 0)
+                                  : iuwt_decompose_partials(g.sub, g.sub, S, 0);
+  z.partial = (size_t)std::max<long>({dec_partials, (long)reduce_grid(N2) * 2, (long)reduce_grid(s2) * S, 4096L}) *
               sizeof(double);
-  z.pmax = (size_t)S * 2048 * sizeof(float);
+  z.pmax = (size_t)S * 2048 * e;
   return z;
 }
 
 static bool fits(const JobSizes& need, const JobSizes& have) {
-  return need.s2 <= have.s2 && need.S_s2 <= have.S_s2 && need.mask <= have.mask && need.iuwt <= have.iuwt &&
-         need.N2 <= have.N2 && need.spec_sub <= have.spec_sub && need.spec_full <= have.spec_full &&
-         need.fft <= have.fft && need.psf <= have.psf && need.ext <= have.ext && need.partial <= have.partial &&
-         need.pmax <= have.pmax;
+  return need.f64 == have.f64 && need.s2 <= have.s2 && need.S_s2 <= have.S_s2 && need.mask <= have.mask &&
+         need.iuwt <= have.iuwt && need.N2 <= have.N2 && need.spec_sub <= have.spec_sub &&
+         need.spec_full <= have.spec_full && need.fft <= have.fft && need.psf <= have.psf && need.ext <= have.ext &&
+         need.partial <= have.partial && need.pmax <= have.pmax;
 }
 
 static JobSizes max_sizes(const JobSizes& a, const JobSizes& b) {
   JobSizes z;
+  z.f64 = a.f64;
   z.s2 = std::max(a.s2, b.s2);
   z.S_s2 = std::max(a.S_s2, b.S_s2);
   z.mask = std::max(a.mask, b.mask);
@@ -425,8 +499,9 @@ static JobSizes max_sizes(const JobSizes& a, const JobSizes& b) {
 
 size_t job_footprint_bytes(const Geometry& g) {
   const JobSizes z = job_sizes(g, std::max(g.S, (int)std::log2((double)g.n) - 1));
+  const size_t by_scale = z.N2;  // moresane_by_scale's private dirty copy
   return z.s2 * 10 + z.S_s2 * 4 + z.mask + z.iuwt + z.N2 * 3 + z.spec_sub + z.spec_full + z.fft + z.psf + z.ext +
-         z.partial + z.pmax + 64 * 8 + 64 * 4 + 12 * 256;
+         z.partial + z.pmax + by_scale + 64 * 8 + 64 * 4 + 12 * 256;
 }
 
 
@@ -451,55 +526,74 @@ Job::~Job() {
   by_scale_arena_.release();
 }
 
-// Buffers grow to the largest geometry seen and are reused by smaller jobs.
+template <class T>
+static void carve(Planes<T>& P, const JobSizes& z, DeviceArena& arena) {
+  using C = typename Planes<T>::C;
+  auto F = [&](size_t bytes) { return (T*)arena.alloc(bytes); };
+  P.dirty_sub = F(z.s2);
+  P.psf_sub = F(z.s2);
+  P.D = F(z.S_s2);
+  P.Dcg = F(z.S_s2);
+  P.E = F(z.S_s2);
+  P.Rs = F(z.S_s2);
+  P.rec = F(z.s2);
+  P.x = F(z.s2);
+  P.xn = F(z.s2);
+  P.r = F(z.s2);
+  P.rn = F(z.s2);
+  P.p = F(z.s2);
+  P.ap = F(z.s2);
+  P.cbuf = F(z.s2);
+  P.iuwt_work = F(z.iuwt);
+  P.model_local = F(z.N2);
+  P.resid = F(z.N2);
+  P.conv_full = F(z.N2);
+  P.spec_sub = (C*)arena.alloc(z.spec_sub);
+  P.spec_full_buf = (C*)arena.alloc(z.spec_full);
+  P.spec_full = nullptr;
+  P.fft_work = (C*)arena.alloc(z.fft);
+  P.psf_frame = F(z.psf);
+  P.pmax = F(z.pmax);
+}
+
+// Buffers grow to the largest geometry seen and are reused by smaller jobs of the same precision.
 bool Job::ensure(const Geometry& g, hipStream_t st) {
   const int S = std::max(g.S, (int)std::log2((double)g.n) - 1);
   const JobSizes need = job_sizes(g, S);
   geo_ = g;
   if (have_ && fits(need, cap_)) return false;
-  const JobSizes z = have_ ? max_sizes(need, cap_) : need;
+  const JobSizes z = (have_ && cap_.f64 == need.f64) ? max_sizes(need, cap_) : need;
   arena_.release();
   cap_ = z;
-  auto F = [&](size_t bytes) { return (float*)arena_.alloc(bytes); };
-  dirty_sub = F(z.s2);
-  psf_sub = F(z.s2);
-  D = F(z.S_s2);
-  Dcg = F(z.S_s2);
-  E = F(z.S_s2);
-  Rs = F(z.S_s2);
+  f = Planes<float>{};
+  d = Planes<double>{};
+  if (g.f64)
+    carve(d, z, arena_);
+  else
+    carve(f, z, arena_);
   M = (uint8_t*)arena_.alloc(z.mask);
-  rec = F(z.s2);
-  x = F(z.s2);
-  xn = F(z.s2);
-  r = F(z.s2);
-  rn = F(z.s2);
-  p = F(z.s2);
-  ap = F(z.s2);
-  cbuf = F(z.s2);
-  iuwt_work = F(z.iuwt);
-  model_local = F(z.N2);
-  resid = F(z.N2);
-  conv_full = F(z.N2);
-  spec_sub = (float2*)arena_.alloc(z.spec_sub);
-  spec_full_buf = (float2*)arena_.alloc(z.spec_full);
-  fft_work = (float2*)arena_.alloc(z.fft);
-  psf_frame = F(z.psf);
   mad_work = arena_.alloc(mad_work_bytes(16));
   mad_work_init(mad_work, 16, st);
   ext_work = arena_.alloc(z.ext);
   partial = (double*)arena_.alloc(z.partial);
-  pmax = (float*)arena_.alloc(z.pmax);
   have_ = true;
+  // the by-scale copy lives in its own arena and survives a regrowth of the others
+  f.cur_dirty = (float*)by_scale_ptr_;
+  d.cur_dirty = (double*)by_scale_ptr_;
   return true;
 }
 
-float* Job::by_scale_dirty(size_t n2) {
-  if (n2 > by_scale_n2_) {
+template <class T>
+T* Job::by_scale_dirty(size_t n2) {
+  const size_t bytes = n2 * sizeof(T);
+  if (bytes > by_scale_bytes_) {
     by_scale_arena_.release();
-    cur_dirty = (float*)by_scale_arena_.alloc(n2 * sizeof(float));
-    by_scale_n2_ = n2;
+    by_scale_ptr_ = by_scale_arena_.alloc(bytes);
+    by_scale_bytes_ = bytes;
   }
-  return cur_dirty;
+  f.cur_dirty = (float*)by_scale_ptr_;
+  d.cur_dirty = (double*)by_scale_ptr_;
+  return (T*)by_scale_ptr_;
 }
 
 // --------------------------------------------------------------------------- task
@@ -525,29 +619,51 @@ void Task::start_call(int scale_count) {
   phase_ = SETUP;
 }
 
+void Task::enqueue_pre(hipStream_t st) {
+  if (f64())
+    enqueue_pre_t<double>(st);
+  else
+    enqueue_pre_t<float>(st);
+}
+
+void Task::add_shared(SharedOps& ops) {
+  if (f64())
+    add_shared_t<double>(ops);
+  else
+    add_shared_t<float>(ops);
+}
+
+void Task::enqueue_post(hipStream_t st) {
+  if (f64())
+    enqueue_post_t<double>(st);
+  else
+    enqueue_post_t<float>(st);
+}
+
 // Step work by phase.  pre: everything up to the shared part; shared: the decomposition of the dirty
 // subregion (DECOMP), the recomposition of the extracted sources (first MINOR step), the full-frame
 // model convolution (UPDATE / REVERT); post: the rest.  Stream order keeps pre < shared < post.
-void Task::enqueue_pre(hipStream_t st) {
+template <class T>
+void Task::enqueue_pre_t(hipStream_t st) {
   if (upload_energies_ > 0) {
     PMS_HIP(hipMemcpyAsync(scal_ + kEnergy, upload_, sizeof(double) * upload_energies_, hipMemcpyHostToDevice, st));
     upload_energies_ = 0;
   }
   ++steps;
   Job& J = *job_;
+  Planes<T>& B = J.pl<T>();
   const int sub = g_.sub;
   const size_t s2 = (size_t)sub * sub;
   switch (phase_) {
-    case SETUP: enqueue_setup(st); break;
+    case SETUP: enqueue_setup<T>(st); break;
     case DECOMP:
       if (spec.mask) {
         // decomposition of dirty_sub * mask[sub] replaces the thresholded one (main.py:283-285)
-        iuwt_decompose(J.dirty_sub, sub, sub, g_.S, 0, J.D, nullptr, J.iuwt_work, nullptr, nullptr, nullptr, nullptr,
-                       st);
-        mad_thresholds(J.D, g_.S, sub, sub, nm_, scal_ + kThr, J.mad_work, nullptr, st);
-        window_copy(spec.mask, g_.n, g_.lo, g_.lo, sub, nullptr, J.cbuf, st);
-        window_copy(J.dirty_sub, sub, 0, 0, sub, J.cbuf, J.rec, st);
-        iuwt_decompose(J.rec, sub, sub, g_.S, 0, J.D, nullptr, J.iuwt_work, nullptr, nullptr, nullptr, nullptr, st);
+        decompose1<T>(B.dirty_sub, sub, g_.S, B.D, B.iuwt_work, st);
+        mad_thresholds(B.D, g_.S, sub, sub, nm_, scal_ + kThr, J.mad_work, nullptr, st);
+        window_copy(spec.mask, g_.n, g_.lo, g_.lo, sub, (const T*)nullptr, B.cbuf, st);
+        window_copy(B.dirty_sub, sub, 0, 0, sub, B.cbuf, B.rec, st);
+        decompose1<T>(B.rec, sub, g_.S, B.D, B.iuwt_work, st);
       }
       break;
     case MINOR:
@@ -555,101 +671,115 @@ void Task::enqueue_pre(hipStream_t st) {
         // source extraction (main.py:347-355); masks/extracted live at planes [adj, max_scale) so the
         // decomposition can index them by scale
         const int adj = scale_adjust_;
-        source_extract(J.D + (size_t)adj * s2, max_scale_ - adj, sub, sub, spec.prm.tolerance, spec.prm.neg_comp,
-                       J.M + (size_t)adj * s2, J.E + (size_t)adj * s2, nullptr, J.ext_work, nullptr, st);
+        source_extract(B.D + (size_t)adj * s2, max_scale_ - adj, sub, sub, spec.prm.tolerance, spec.prm.neg_comp,
+                       J.M + (size_t)adj * s2, B.E + (size_t)adj * s2, nullptr, J.ext_work, nullptr, st);
       }
       break;
     case UPDATE:
     case REVERT: {
       // model += gain x (a negative gain reverts the update), main.py:479-497
       const double gain = phase_ == UPDATE ? spec.prm.loop_gain : -spec.prm.loop_gain;
-      hipLaunchKernelGGL(k_model_add, dim3(grid_for(s2)), dim3(kT), 0, st, J.model_local, g_.n, g_.lo, g_.lo, sub,
-                         (const float*)x_, gain);
+      hipLaunchKernelGGL(k_model_add<T>, dim3(grid_for(s2)), dim3(kT), 0, st, B.model_local, g_.n, g_.lo, g_.lo, sub,
+                         (const T*)x_, gain);
       PMS_LAUNCH_CHECK();
       break;
     }
-    case FINISH: enqueue_finish(st); break;
+    case FINISH: enqueue_finish<T>(st); break;
     case DONE: break;
   }
 }
 
-void Task::add_shared(SharedOps& ops) {
-  Job& J = *job_;
+template <class T>
+void Task::add_shared_t(SharedOps& all) {
+  SharedOpsT<T>& ops = all.of<T>();
+  Planes<T>& B = job_->pl<T>();
   const int sub = g_.sub;
   const size_t s2 = (size_t)sub * sub;
   if (phase_ == DECOMP && !spec.mask) {
-    ops.decomp.push_back({sub, IuwtDecompJob{J.dirty_sub, g_.S, 0, J.D, nullptr, J.iuwt_work, nullptr, nullptr,
-                                             nullptr, nullptr, 0}});
+    ops.decomp.push_back({sub, IuwtDecompJobT<T>{B.dirty_sub, g_.S, 0, B.D, nullptr, B.iuwt_work, nullptr, nullptr,
+                                                 nullptr, nullptr, 0}});
   } else if (phase_ == MINOR && minor_first_) {
     const int adj = scale_adjust_;
-    ops.recomp.push_back({sub, IuwtRecompJob{J.E + (size_t)adj * s2, max_scale_ - adj, adj, nullptr, J.rec,
-                                             J.iuwt_work, nullptr, nullptr}});
+    ops.recomp.push_back({sub, IuwtRecompJobT<T>{B.E + (size_t)adj * s2, max_scale_ - adj, adj, nullptr, B.rec,
+                                                 B.iuwt_work, nullptr, nullptr}});
   } else if (phase_ == UPDATE || phase_ == REVERT) {
-    ops.conv.push_back({{g_.n, g_.p_full}, FftConvJob{J.model_local, J.spec_full, J.conv_full, J.fft_work, nullptr}});
+    ops.conv.push_back(
+        {{g_.n, g_.p_full}, typename EngineTypes<T>::Conv{B.model_local, B.spec_full, B.conv_full, B.fft_work, nullptr}});
   }
 }
 
-void Task::enqueue_post(hipStream_t st) {
+template <class T>
+void Task::enqueue_post_t(hipStream_t st) {
   const MoresaneParams& prm = spec.prm;
   Job& J = *job_;
+  Planes<T>& B = J.pl<T>();
   const int sub = g_.sub;
   const size_t s2 = (size_t)sub * sub;
   const size_t N2 = (size_t)g_.n * g_.n;
   switch (phase_) {
     case DECOMP:
       // MAD thresholds, thresholding with the edge mask and normalised maxima (main.py:278-300)
-      if (!spec.mask) mad_thresholds(J.D, g_.S, sub, sub, nm_, scal_ + kThr, J.mad_work, nullptr, st);
-      apply_threshold(J.D, J.D, g_.S, sub, sub, scal_ + kThr, prm.sigma_level,
-                      borders_.empty() ? nullptr : iflag_ + kBorders, J.pmax, nullptr, st);
-      max_final(J.pmax, threshold_grid(s2), g_.S, nullptr, scal_ + kEnergy, scal_ + kNsm, nullptr, st);
+      if (!spec.mask) mad_thresholds(B.D, g_.S, sub, sub, nm_, scal_ + kThr, J.mad_work, nullptr, st);
+      apply_threshold(B.D, B.D, g_.S, sub, sub, scal_ + kThr, prm.sigma_level,
+                      borders_.empty() ? nullptr : iflag_ + kBorders, B.pmax, nullptr, st);
+      max_final(B.pmax, threshold_grid(s2), g_.S, (T*)nullptr, scal_ + kEnergy, scal_ + kNsm, nullptr, st);
       break;
     case MINOR:
       if (minor_first_) {
         // CG set-up (main.py:365-377)
         const int ns = max_scale_ - scale_adjust_;
-        const float* Es = J.E + (size_t)scale_adjust_ * s2;
-        fill_f32(x_, s2, 0.f, st);
+        const T* Es = B.E + (size_t)scale_adjust_ * s2;
+        fill_zero((T*)x_, s2, st);
         // the incremental SNR's difference planes start at E - M.W(H x) = E (x = 0)
-        copy_plane(Es, J.Rs, (size_t)ns * s2, nullptr, st);
-        copy_plane(J.rec, r_, s2, nullptr, st);
-        copy_plane(J.rec, J.p, s2, nullptr, st);
-        dot_partials(r_, nullptr, s2, J.partial, nullptr, st);
+        copy_plane(Es, B.Rs, (size_t)ns * s2, nullptr, st);
+        copy_plane(B.rec, (T*)r_, s2, nullptr, st);
+        copy_plane(B.rec, B.p, s2, nullptr, st);
+        dot_partials((const T*)r_, (const T*)nullptr, s2, J.partial, nullptr, st);
         sum_partials(J.partial, reduce_grid(s2), 1, rr_, nullptr, st);
         plane_sumsq_partials(Es, 1, (size_t)ns * s2, J.partial, nullptr, st);
         sum_partials(J.partial, reduce_grid((size_t)ns * s2), 1, scal_ + kE2, nullptr, st);
       }
       break;
     case UPDATE:
-    case REVERT:
+    case REVERT: {
       // residual = dirty - H model, std of residual[sub], max of residual[sub] (main.py:497-506)
-      hipLaunchKernelGGL(k_sub, dim3(grid_for(N2)), dim3(kT), 0, st, dirty_dev(), (const float*)J.conv_full, J.resid,
-                         N2);
+      if (spec.by_scale)
+        hipLaunchKernelGGL((k_sub<T, T>), dim3(grid_for(N2)), dim3(kT), 0, st, (const T*)B.cur_dirty,
+                           (const T*)B.conv_full, B.resid, N2);
+      else
+        hipLaunchKernelGGL((k_sub<float, T>), dim3(grid_for(N2)), dim3(kT), 0, st, spec.dirty, (const T*)B.conv_full,
+                           B.resid, N2);
       PMS_LAUNCH_CHECK();
-      window_copy(J.resid, g_.n, g_.lo, g_.lo, sub, nullptr, J.dirty_sub, st);
-      mean_std(J.dirty_sub, s2, J.partial, scal_ + kMeanSum, scal_ + kDev2, st);
-      plane_max(J.dirty_sub, 1, s2, 0, J.pmax, nullptr, st);
-      max_final(J.pmax, threshold_grid(s2), 1, (float*)(scal_ + kDmax), nullptr, nullptr, nullptr, st);
+      window_copy(B.resid, g_.n, g_.lo, g_.lo, sub, (const T*)nullptr, B.dirty_sub, st);
+      mean_std(B.dirty_sub, s2, J.partial, scal_ + kMeanSum, scal_ + kDev2, st);
+      plane_max(B.dirty_sub, 1, s2, 0, B.pmax, nullptr, st);
+      max_final(B.pmax, threshold_grid(s2), 1, (T*)(scal_ + kDmax), nullptr, nullptr, nullptr, st);
       break;
+    }
     default: break;
   }
 }
 
 // psf_sub, PSF spectra and per-scale PSF energies (main.py:126-226), the dirty subregion and its
 // maximum (main.py:229-262).
+template <class T>
 void Task::enqueue_setup(hipStream_t st) {
   const MoresaneParams& prm = spec.prm;
   int sub = 0;
   Geometry g = make_geometry(prm.n, prm.psf_n, prm, scale_count_, &sub);
-  if (job_->ensure(g, st) || !x_) {
+  Job& J = *job_;
+  if (J.ensure(g, st) || !x_) {
     moved_ = true;
-    x_ = job_->x;
-    xn_ = job_->xn;
-    r_ = job_->r;
-    rn_ = job_->rn;
+    Planes<T>& P = J.pl<T>();
+    x_ = P.x;
+    xn_ = P.xn;
+    r_ = P.r;
+    rn_ = P.rn;
     rr_ = scal_ + kRR;
     rnrn_ = scal_ + kRNRN;
   }
-  g_ = job_->geometry();
+  Planes<T>& B = J.pl<T>();
+  g_ = J.geometry();
   const int S = g_.S;
   PMS_REQUIRE(S <= 16, "scale_count above 16 is not supported");
   const size_t N2 = (size_t)g_.n * g_.n;
@@ -657,44 +787,45 @@ void Task::enqueue_setup(hipStream_t st) {
   if (spec.by_scale && !by_scale_copied_) {
     by_scale_copied_ = true;
     // the dirty image of each call is the previous residual; a private copy keeps the caller's intact
-    copy_plane(spec.dirty, job_->by_scale_dirty(N2), N2, nullptr, st);
+    T* cd = J.by_scale_dirty<T>(N2);
+    window_copy(spec.dirty, g_.n, 0, 0, g_.n, (const T*)nullptr, cd, st);
   }
-  Job& J = *job_;
-  J.spec_full = (g_.p_full == g_.p_sub && g_.sub == g_.n) ? J.spec_sub : J.spec_full_buf;
+  B.spec_full = (g_.p_full == g_.p_sub && g_.sub == g_.n) ? B.spec_sub : B.spec_full_buf;
   const float* psf_dev = spec.psf;
   const int pn = g_.psf_n;
   const int plo = g_.double_psf ? pn / 2 - g_.sub / 2 : g_.lo;
-  window_copy(psf_dev, pn, plo, plo, g_.sub, nullptr, J.psf_sub, st);
-  // spectra: the frame the reference transforms, expressed as (source window, q) for fft_psf_spectrum
-  auto spectrum_of = [&](int lo, int q, int p, float2* dst) {
-    const float* src = psf_dev;
-    if (!(lo == 0 && q == pn)) {
-      window_copy(psf_dev, pn, lo, lo, q, nullptr, J.psf_frame, st);
-      src = J.psf_frame;
+  window_copy(psf_dev, pn, plo, plo, g_.sub, (const T*)nullptr, B.psf_sub, st);
+  // spectra: the frame the reference transforms, expressed as (source window, q) for the spectrum
+  using C = typename Planes<T>::C;
+  auto spectrum_of = [&](int lo, int q, int p, C* dst) {
+    if (lo == 0 && q == pn) {
+      psf_spectrum_of(psf_dev, q, p, dst, B.fft_work, st);
+    } else {
+      window_copy(psf_dev, pn, lo, lo, q, (const T*)nullptr, B.psf_frame, st);
+      psf_spectrum_of((const T*)B.psf_frame, q, p, dst, B.fft_work, st);
     }
-    fft_psf_spectrum(src, q, p, dst, J.fft_work, st);
   };
   const bool full = g_.sub == g_.n;
   if (g_.linear) {
     // a 2N PSF is cropped to the frame; an N x N PSF is zero-padded to 2N (pad_array): q = N
     if (full) {
-      spectrum_of(0, pn, g_.p_full, J.spec_sub);
+      spectrum_of(0, pn, g_.p_full, B.spec_sub);
     } else {
-      spectrum_of(pn / 2 - g_.sub, 2 * g_.sub, g_.p_sub, J.spec_sub);
-      spectrum_of(0, pn, g_.p_full, J.spec_full);
+      spectrum_of(pn / 2 - g_.sub, 2 * g_.sub, g_.p_sub, B.spec_sub);
+      spectrum_of(0, pn, g_.p_full, B.spec_full);
     }
   } else {
     if (g_.double_psf) {
-      spectrum_of(pn / 2 - g_.sub / 2, g_.sub, g_.p_sub, J.spec_sub);
-      if (J.spec_full != J.spec_sub) spectrum_of(pn / 2 - g_.n / 2, g_.n, g_.p_full, J.spec_full);
+      spectrum_of(pn / 2 - g_.sub / 2, g_.sub, g_.p_sub, B.spec_sub);
+      if (B.spec_full != B.spec_sub) spectrum_of(pn / 2 - g_.n / 2, g_.n, g_.p_full, B.spec_full);
     } else {
-      spectrum_of(g_.lo, g_.sub, g_.p_sub, J.spec_sub);
-      if (J.spec_full != J.spec_sub) spectrum_of(0, pn, g_.p_full, J.spec_full);
+      spectrum_of(g_.lo, g_.sub, g_.p_sub, B.spec_sub);
+      if (B.spec_full != B.spec_sub) spectrum_of(0, pn, g_.p_full, B.spec_full);
     }
   }
   // squared energies of the PSF decomposition; their square roots come back as an upload
-  iuwt_decompose(J.psf_sub, g_.sub, g_.sub, S, 0, J.D, nullptr, J.iuwt_work, nullptr, nullptr, nullptr, nullptr, st);
-  plane_sumsq_partials(J.D, S, s2, J.partial, nullptr, st);
+  decompose1<T>(B.psf_sub, g_.sub, S, B.D, B.iuwt_work, st);
+  plane_sumsq_partials(B.D, S, s2, J.partial, nullptr, st);
   sum_partials(J.partial, reduce_grid(s2), S, scal_ + kEnergy, nullptr, st);
   nm_ = noise_mask(g_.sub, g_.sub, prm.edge_excl, prm.int_excl);
   borders_ = suppression_borders(S, prm.edge_suppression, prm.edge_offset);
@@ -704,39 +835,63 @@ void Task::enqueue_setup(hipStream_t st) {
     PMS_HIP(hipMemcpyAsync(iflag_ + kBorders, ub, sizeof(int) * S, hipMemcpyHostToDevice, st));
   }
   // model (local to this call) starts at zero; dirty subregion window and its maximum
-  fill_f32(J.model_local, N2, 0.f, st);
-  window_copy(dirty_dev(), g_.n, g_.lo, g_.lo, g_.sub, nullptr, J.dirty_sub, st);
-  plane_max(J.dirty_sub, 1, s2, 0, J.pmax, nullptr, st);
-  max_final(J.pmax, threshold_grid(s2), 1, (float*)(scal_ + kDmax), nullptr, nullptr, nullptr, st);
+  fill_zero(B.model_local, N2, st);
+  if (spec.by_scale)
+    window_copy((const T*)B.cur_dirty, g_.n, g_.lo, g_.lo, g_.sub, (const T*)nullptr, B.dirty_sub, st);
+  else
+    window_copy(spec.dirty, g_.n, g_.lo, g_.lo, g_.sub, (const T*)nullptr, B.dirty_sub, st);
+  plane_max(B.dirty_sub, 1, s2, 0, B.pmax, nullptr, st);
+  max_final(B.pmax, threshold_grid(s2), 1, (T*)(scal_ + kDmax), nullptr, nullptr, nullptr, st);
 }
 
-
-
-
+template <class T>
 void Task::enqueue_finish(hipStream_t st) {
-  Job& J = *job_;
+  Planes<T>& B = job_->pl<T>();
   const size_t N2 = (size_t)g_.n * g_.n;
   if (major_ > 0) {
-    hipLaunchKernelGGL(k_add_inplace, dim3(grid_for(N2)), dim3(kT), 0, st, spec.model, (const float*)J.model_local,
-                       N2);
+    hipLaunchKernelGGL((k_add_inplace<float, T>), dim3(grid_for(N2)), dim3(kT), 0, st, spec.model,
+                       (const T*)B.model_local, N2);
     PMS_LAUNCH_CHECK();
-    copy_plane(J.resid, spec.residual, N2, nullptr, st);
+    if constexpr (sizeof(T) == sizeof(float)) {
+      copy_plane(B.resid, spec.residual, N2, nullptr, st);
+    } else {
+      hipLaunchKernelGGL(k_to_f32, dim3(grid_for(N2)), dim3(kT), 0, st, (const double*)B.resid, spec.residual, N2);
+      PMS_LAUNCH_CHECK();
+    }
   }
-  if (spec.by_scale) copy_plane(spec.residual, J.cur_dirty, N2, nullptr, st);
+  // moresane_by_scale: the residual is the next call's dirty image (main.py:584), kept in the job's
+  // precision; a call that did no work leaves the residual as it was
+  if (spec.by_scale) {
+    if (major_ > 0)
+      copy_plane((const T*)B.resid, B.cur_dirty, N2, nullptr, st);
+    else
+      window_copy((const float*)spec.residual, g_.n, 0, 0, g_.n, (const T*)nullptr, B.cur_dirty, st);
+  }
 }
 
 IterArgs Task::iter_args() const {
-  return IterArgs{x_, xn_, r_, rn_, job_->p, job_->ap, rr_, rnrn_, max_scale_, scale_adjust_,
-                  spec.prm.enforce_positivity, g_.sub, g_.p_sub};
+  const int f = f64() ? 1 : 0;
+  void* p = f ? (void*)job_->d.p : (void*)job_->f.p;
+  void* ap = f ? (void*)job_->d.ap : (void*)job_->f.ap;
+  return IterArgs{x_, xn_, r_, rn_, p, ap, rr_, rnrn_, max_scale_, scale_adjust_, spec.prm.enforce_positivity,
+                  g_.sub, g_.p_sub, f};
 }
 
 // ---- host decisions
+// the subregion maximum, stored in the slot as the job's element type
+double Task::read_dmax(const double* h) const {
+  if (f64()) return h[kDmax];
+  float v;
+  std::memcpy(&v, h + kDmax, sizeof(float));
+  return (double)v;
+}
+
 void Task::consume(const double* h) {
   switch (phase_) {
     case SETUP: {
       for (int i = 0; i < g_.S; ++i) upload_[i] = std::sqrt(h[kEnergy + i]);
       upload_energies_ = g_.S;
-      std::memcpy(&dmax_, h + kDmax, sizeof(float));
+      dmax_ = read_dmax(h);
       major_ = 0;
       max_coeff_ = 1.0;
       std_current_ = 1000.0;
@@ -806,14 +961,14 @@ void Task::consume(const double* h) {
         break;
       }
       trace.push_back(ev(EV_UPDATE, major_, 0, 0, 0, std_current_, std_ratio_));
-      std::memcpy(&dmax_, h + kDmax, sizeof(float));
+      dmax_ = read_dmax(h);
       ++major_;
       major_check();
       break;
     }
     case REVERT:
       trace.push_back(ev(EV_UPDATE, major_, 1, 0, 0, std_current_, std_ratio_));
-      std::memcpy(&dmax_, h + kDmax, sizeof(float));
+      dmax_ = read_dmax(h);
       ++major_;
       major_check();
       break;
@@ -840,7 +995,7 @@ void Task::consume(const double* h) {
 void Task::major_check() {
   const MoresaneParams& prm = spec.prm;
   if (major_ < prm.major_loop_miter && max_coeff_ > 0 && std_ratio_ > prm.accuracy &&
-      (double)dmax_ > prm.flux_threshold) {
+      dmax_ > prm.flux_threshold) {
     max_scale_ = 0;
     scale_adjust_ = 0;
     inner_check();
@@ -1056,29 +1211,35 @@ void Runner::step() {
   }
 }
 
-// The shared part of every task's step, one batched call per plane side / frame.
+// The shared part of every task's step, one batched call per plane side / frame and element type.
 void Runner::run_shared(const std::vector<int>& active) {
   SharedOps ops;
   for (int k : active)
     if (!tasks_[k]->done()) tasks_[k]->add_shared(ops);
-  std::map<int, std::vector<IuwtDecompJob>> dec;
-  std::map<int, std::vector<IuwtRecompJob>> rec;
-  std::map<std::pair<int, int>, std::vector<FftConvJob>> conv;
+  run_shared_t<float>(ops.f);
+  run_shared_t<double>(ops.d);
+}
+
+template <class T>
+void Runner::run_shared_t(SharedOpsT<T>& ops) {
+  std::map<int, std::vector<IuwtDecompJobT<T>>> dec;
+  std::map<int, std::vector<IuwtRecompJobT<T>>> rec;
+  std::map<std::pair<int, int>, std::vector<typename EngineTypes<T>::Conv>> conv;
   for (auto& e : ops.decomp) dec[e.first].push_back(e.second);
   for (auto& e : ops.recomp) rec[e.first].push_back(e.second);
   for (auto& e : ops.conv) conv[e.first].push_back(e.second);
-  for (auto& kv : dec) iuwt_decompose_batch(kv.second.data(), (int)kv.second.size(), kv.first, kv.first, st_);
-  for (auto& kv : rec) iuwt_recompose_batch(kv.second.data(), (int)kv.second.size(), kv.first, kv.first, st_);
-  for (auto& kv : conv)
-    fft_conv_batch(kv.second.data(), (int)kv.second.size(), kv.first.first, kv.first.second, st_);
+  for (auto& kv : dec) decompose_batch(kv.second.data(), (int)kv.second.size(), kv.first, kv.first, st_);
+  for (auto& kv : rec) recompose_batch(kv.second.data(), (int)kv.second.size(), kv.first, kv.first, st_);
+  for (auto& kv : conv) conv_batch(kv.second.data(), (int)kv.second.size(), kv.first.first, kv.first.second, st_);
 }
 
-// Iterations of tasks that share a plane shape and FFT frame run together, kMaxBatch per launch.
+// Iterations of tasks that share an element type, plane shape and FFT frame run together, kMaxBatch
+// per launch.
 void Runner::run_iterations(std::vector<Task*>& ts) {
-  std::map<std::pair<int, int>, std::vector<Task*>> parts;
+  std::map<std::tuple<int, int, int>, std::vector<Task*>> parts;
   for (Task* t : ts) {
     const IterArgs a = t->iter_args();
-    parts[{a.sub, a.p_sub}].push_back(t);
+    parts[std::make_tuple(a.f64, a.sub, a.p_sub)].push_back(t);
   }
   for (auto& kv : parts) {
     std::vector<Task*>& v = kv.second;
@@ -1123,86 +1284,99 @@ void Runner::drop_graphs() {
 
 // Ap <- R(M . W(H v)) over scales [adj, max_scale) for every task (main.py:380-390); the clip pass
 // recomputes it for tasks whose positivity clip fired (predicated on the device flag).
+template <class T>
 void Runner::apply_op(const std::vector<Task*>& ts, const std::vector<IterArgs>& its, bool clip_pass) {
   const int sub = its[0].sub, p_sub = its[0].p_sub;
   const size_t s2 = (size_t)sub * sub;
-  std::vector<FftConvJob> fj;
-  std::vector<IuwtDecompJob> dj;
-  std::vector<IuwtRecompJob> rj;
+  std::vector<typename EngineTypes<T>::Conv> fj;
+  std::vector<IuwtDecompJobT<T>> dj;
+  std::vector<IuwtRecompJobT<T>> rj;
   for (size_t i = 0; i < ts.size(); ++i) {
     Job& J = *ts[i]->job();
+    Planes<T>& B = J.pl<T>();
     const IterArgs& a = its[i];
     const int* pred = clip_pass ? ts[i]->iflag() + kClip : nullptr;
-    fj.push_back(FftConvJob{a.p, J.spec_sub, J.cbuf, J.fft_work, pred});
-    dj.push_back(IuwtDecompJob{J.cbuf, a.max_scale, a.adj, J.Dcg, nullptr, J.iuwt_work, J.M + (size_t)a.adj * s2,
-                               nullptr, nullptr, pred, 0});
-    rj.push_back(IuwtRecompJob{J.Dcg, a.max_scale - a.adj, a.adj, nullptr, a.ap, J.iuwt_work, nullptr, pred});
+    fj.push_back(typename EngineTypes<T>::Conv{(const T*)a.p, B.spec_sub, B.cbuf, B.fft_work, pred});
+    dj.push_back(IuwtDecompJobT<T>{B.cbuf, a.max_scale, a.adj, B.Dcg, nullptr, B.iuwt_work, J.M + (size_t)a.adj * s2,
+                                   nullptr, nullptr, pred, 0});
+    rj.push_back(IuwtRecompJobT<T>{B.Dcg, a.max_scale - a.adj, a.adj, nullptr, (T*)a.ap, B.iuwt_work, nullptr, pred});
   }
-  fft_conv_batch(fj.data(), (int)fj.size(), sub, p_sub, st_);
-  iuwt_decompose_batch(dj.data(), (int)dj.size(), sub, sub, st_);
-  iuwt_recompose_batch(rj.data(), (int)rj.size(), sub, sub, st_);
+  conv_batch(fj.data(), (int)fj.size(), sub, p_sub, st_);
+  decompose_batch(dj.data(), (int)dj.size(), sub, sub, st_);
+  recompose_batch(rj.data(), (int)rj.size(), sub, sub, st_);
 }
 
-// One CG iteration (main.py:380-424) up to the SNR sums for up to kMaxBatch tasks.
 void Runner::enqueue_iterations(const std::vector<IterArgs>& its, const std::vector<Task*>& ts) {
+  if (its[0].f64)
+    enqueue_iterations_t<double>(its, ts);
+  else
+    enqueue_iterations_t<float>(its, ts);
+}
+
+// One CG iteration (main.py:380-424) up to the SNR sums for up to kMaxBatch tasks of one element type.
+template <class T>
+void Runner::enqueue_iterations_t(const std::vector<IterArgs>& its, const std::vector<Task*>& ts) {
   const int nb = (int)ts.size();
   const int sub = its[0].sub, p_sub = its[0].p_sub;
   const size_t s2 = (size_t)sub * sub;
-  CgBatch cb{};
+  CgBatch<T> cb{};
   cb.n = s2;
   std::vector<Task*> clip_ts;
   std::vector<IterArgs> clip_its;
-  CgBatch clip_b{};
+  CgBatch<T> clip_b{};
   clip_b.n = s2;
   for (int i = 0; i < nb; ++i) {
     const IterArgs& a = its[i];
     Job& J = *ts[i]->job();
-    cb.j[i] = CgJob{a.x, a.xn, a.r, a.rn, a.p, a.ap, J.partial, ts[i]->scal(), a.rr, a.rnrn, ts[i]->iflag(),
-                    a.enforce};
+    cb.j[i] = CgJob<T>{(T*)a.x,     (T*)a.xn,      (T*)a.r, (T*)a.rn,        (T*)a.p,
+                       (T*)a.ap,    J.partial,     ts[i]->scal(), a.rr,       a.rnrn,
+                       ts[i]->iflag(), a.enforce};
     if (a.enforce) {
       clip_b.j[clip_ts.size()] = cb.j[i];
       clip_ts.push_back(ts[i]);
       clip_its.push_back(a);
     }
   }
-  apply_op(ts, its, false);
-  launch_cg(k_cg_pap, cb, nb, st_);
-  launch_cg_fin(k_cg_alpha_fin, cb, nb, st_);
-  launch_cg(k_cg_xn, cb, nb, st_);
+  apply_op<T>(ts, its, false);
+  launch_cg(k_cg_pap<T>, cb, nb, st_);
+  launch_cg_fin(k_cg_alpha_fin<T>, cb, nb, st_);
+  launch_cg(k_cg_xn<T>, cb, nb, st_);
   if (!clip_ts.empty()) {
-    launch_cg_after(k_cg_clip, clip_b, (int)clip_ts.size(), st_);
-    apply_op(clip_ts, clip_its, true);
+    launch_cg_after(k_cg_clip<T>, clip_b, (int)clip_ts.size(), st_);
+    apply_op<T>(clip_ts, clip_its, true);
   }
-  launch_cg(k_cg_rn, cb, nb, st_);
-  launch_cg_after(k_cg_p, cb, nb, st_);
+  launch_cg(k_cg_rn<T>, cb, nb, st_);
+  launch_cg_after(k_cg_p<T>, cb, nb, st_);
   // model sources and the SNR sums (main.py:413-424)
   std::vector<SumJob> sj;
   if (snr_direct_) {
-    std::vector<FftConvJob> fj;
-    std::vector<IuwtDecompJob> dj;
+    std::vector<typename EngineTypes<T>::Conv> fj;
+    std::vector<IuwtDecompJobT<T>> dj;
     for (int i = 0; i < nb; ++i) {
       Job& J = *ts[i]->job();
+      Planes<T>& B = J.pl<T>();
       const IterArgs& a = its[i];
-      fj.push_back(FftConvJob{a.xn, J.spec_sub, J.cbuf, J.fft_work, nullptr});
-      dj.push_back(IuwtDecompJob{J.cbuf, a.max_scale, a.adj, nullptr, nullptr, J.iuwt_work, J.M + (size_t)a.adj * s2,
-                                 J.E + (size_t)a.adj * s2, J.partial, nullptr, 0});
+      fj.push_back(typename EngineTypes<T>::Conv{(const T*)a.xn, B.spec_sub, B.cbuf, B.fft_work, nullptr});
+      dj.push_back(IuwtDecompJobT<T>{B.cbuf, a.max_scale, a.adj, nullptr, nullptr, B.iuwt_work,
+                                     J.M + (size_t)a.adj * s2, B.E + (size_t)a.adj * s2, J.partial, nullptr, 0});
     }
-    fft_conv_batch(fj.data(), nb, sub, p_sub, st_);
-    iuwt_decompose_batch(dj.data(), nb, sub, sub, st_);
+    conv_batch(fj.data(), nb, sub, p_sub, st_);
+    decompose_batch(dj.data(), nb, sub, sub, st_);
     for (int i = 0; i < nb; ++i) sj.push_back(SumJob{ts[i]->job()->partial, dj[i].np, ts[i]->scal() + kD2});
   } else {
-    SnrBatch sb{};
+    SnrBatch<T> sb{};
     int gmax = 1;
     for (int i = 0; i < nb; ++i) {
       Job& J = *ts[i]->job();
+      Planes<T>& B = J.pl<T>();
       const IterArgs& a = its[i];
       const size_t n = (size_t)(a.max_scale - a.adj) * s2;
       const int nblk = cg_grid(n);
-      sb.j[i] = SnrJob{J.Rs, J.Dcg, ts[i]->scal(), J.partial, n, nblk};
+      sb.j[i] = SnrJob<T>{B.Rs, B.Dcg, ts[i]->scal(), J.partial, n, nblk};
       gmax = std::max(gmax, nblk);
       sj.push_back(SumJob{J.partial, nblk, ts[i]->scal() + kD2});
     }
-    hipLaunchKernelGGL(k_cg_snr, dim3(gmax, nb), dim3(kT), 0, st_, sb);
+    hipLaunchKernelGGL(k_cg_snr<T>, dim3(gmax, nb), dim3(kT), 0, st_, sb);
     PMS_LAUNCH_CHECK();
   }
   sum_partials_batch(sj.data(), nb, st_);

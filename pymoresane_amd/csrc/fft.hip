@@ -955,7 +955,7 @@ const float2* fft_twiddles() {
   return g_tw[dev].ptr;
 }
 
-int fft_perm(int k, int p) { return perm_pos(k, ilog2(p)); }
+int fft_perm(int k, int p) { return is_pow2(p) ? perm_pos(k, ilog2(p)) : gfft_perm(k, p); }
 
 size_t fft_spectrum_elems(int p) { return (size_t)(p / 2 + 1) * p; }
 
@@ -968,6 +968,12 @@ static void check_fft_len(int p) {
 }
 
 void fft_psf_spectrum(const float* psf, int q, int p, float2* spec, float2* work, hipStream_t st) {
+  if (!is_pow2(p)) {
+    // any other even side: the mixed-radix transforms (same layout, digit-reversed columns)
+    gfft_psf_spectrum<float, float>(psf, q, p, reinterpret_cast<Cx<float>*>(spec), reinterpret_cast<Cx<float>*>(work),
+                                    st);
+    return;
+  }
   check_fft_len(p);
   PMS_REQUIRE(q >= 1 && q <= p, "PSF larger than the FFT frame");
   const float2* tw = fft_twiddles();
@@ -994,6 +1000,14 @@ void fft_conv(const float* in, int n, int p, const float2* spec, float* out, flo
 // Jobs are independent: job j's blocks read and write only job j's buffers, so a batch gives each
 // job exactly the result of its own single-job call.
 void fft_conv_batch(const FftConvJob* jobs, int count, int n, int p, hipStream_t st) {
+  if (!is_pow2(p)) {
+    std::vector<GConvJob<float>> g(count);
+    for (int k = 0; k < count; ++k)
+      g[k] = GConvJob<float>{jobs[k].in, reinterpret_cast<const Cx<float>*>(jobs[k].spec), jobs[k].out,
+                             reinterpret_cast<Cx<float>*>(jobs[k].work), jobs[k].pred};
+    gfft_conv_batch<float>(g.data(), count, n, p, st);
+    return;
+  }
   check_fft_len(p);
   PMS_REQUIRE(p == n || p == 2 * n, "fft_conv: frame must be N (circular) or 2N (linear)");
   const float2* tw = fft_twiddles();
@@ -1032,11 +1046,13 @@ void fft_conv_batch(const FftConvJob* jobs, int count, int n, int p, hipStream_t
 // unrolled PSF frame -> library layout: column k, position perm(m), times (-1)^(m+k)
 // (the P/2 roll in both axes).
 void fft_spectrum_from_numpy_host(const float* re_im, int p, float2* spec_host) {
-  check_fft_len(p);
-  const int pm = ilog2(p);
+  if (is_pow2(p))
+    check_fft_len(p);
+  else
+    PMS_REQUIRE(gfft_frame_ok(p, sizeof(float)), "FFT frame side " + std::to_string(p) + " is not supported");
   const int nk = p / 2 + 1;
   for (int m = 0; m < p; ++m) {
-    const int pos = perm_pos(m, pm);
+    const int pos = fft_perm(m, p);
     for (int k = 0; k < nk; ++k) {
       const float sgn = ((m + k) & 1) ? -1.f : 1.f;
       const float* v = re_im + 2 * ((size_t)m * nk + k);

@@ -815,13 +815,12 @@ int walk_flags(const IuwtScaleCall& c) {
 
 }  // namespace
 
+// Scales with 2^(s+1) above a side are valid (the reference reads the axis reversed there); they
+// run on the direct-stencil kernels of iuwt_any.hip, the row walks take the rest.
 void check_iuwt_scale(int h, int w, int scale) {
   PMS_REQUIRE(h >= 2 && w >= 2, "IUWT planes must be at least 2x2");
   PMS_REQUIRE(scale >= 0 && scale < 30, "IUWT scale out of range");
   PMS_REQUIRE((long)h * w < (1L << 29), "IUWT planes are limited to 2^29 pixels (32-bit byte offsets)");
-  PMS_REQUIRE((2L << scale) <= h && (2L << scale) <= w,
-              "IUWT scale " + std::to_string(scale) + " needs 2^(scale+1) <= image side (" +
-                  std::to_string(h) + "x" + std::to_string(w) + ")");
 }
 
 bool use_small(int h, int w) { return (long)h * w <= tuning().small_pixels; }
@@ -848,6 +847,7 @@ int iuwt_scale_batch(const IuwtScaleCall* calls, int count, hipStream_t st) {
   PMS_REQUIRE(count >= 1, "empty IUWT batch");
   const IuwtScaleCall& c0 = calls[0];
   check_iuwt_scale(c0.h, c0.w, c0.scale);
+  if (iuwt_over_long(c0.h, c0.w, c0.scale)) return iuwt_gen_scale_batch<float>(calls, count, st);
   for (int k = 1; k < count; ++k)
     PMS_REQUIRE(calls[k].mode == c0.mode && calls[k].h == c0.h && calls[k].w == c0.w && calls[k].scale == c0.scale,
                 "IUWT batch mixes modes, shapes or scales");
@@ -910,6 +910,7 @@ int iuwt_scale_batch(const IuwtScaleCall* calls, int count, hipStream_t st) {
 }
 
 int iuwt_scale_blocks(int h, int w, int scale, int mode) {
+  if (iuwt_over_long(h, w, scale)) return iuwt_gen_blocks(h, w);
   if (use_small(h, w)) {
     const dim3 g = small_grid(h, w);
     return (int)(g.x * g.y);
@@ -972,7 +973,8 @@ static void decompose_batch_impl(IuwtDecompJob* jobs, int count, int h, int w, h
           if (s < j.scale_adjust || s >= j.scale_count) continue;
           const int kk = s - j.scale_adjust;
           const bool last = (s == j.scale_count - 1);
-          c.out_c = last ? j.smoothed : ping;
+          // the direct-stencil form of an over-long scale needs C even when it is not kept
+          c.out_c = (last && (j.smoothed || !iuwt_over_long(h, w, s))) ? j.smoothed : ping;
           c.out_w = j.detail ? j.detail + (size_t)kk * plane : nullptr;
           c.mask = j.mask ? j.mask + (size_t)kk * plane : nullptr;
           c.ref = j.ref ? j.ref + (size_t)kk * plane : nullptr;

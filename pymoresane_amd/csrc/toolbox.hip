@@ -50,7 +50,8 @@ __global__ __launch_bounds__(kT) void k_copy(const float* __restrict__ src, floa
   for (size_t t = (n4 << 2) + blockIdx.x * (size_t)kT + threadIdx.x; t < n; t += stride) dst[t] = src[t];
 }
 
-__global__ void k_copy1(const float* __restrict__ src, float* __restrict__ dst, size_t n, const int* pred) {
+template <class T>
+__global__ void k_copy1(const T* __restrict__ src, T* __restrict__ dst, size_t n, const int* pred) {
   if (pred_off(pred)) return;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     dst[i] = src[i];
@@ -68,11 +69,12 @@ __global__ __launch_bounds__(kT) void k_fill16(T* __restrict__ p, size_t n, T v)
   for (size_t t = (n4 << 2) + blockIdx.x * (size_t)kT + threadIdx.x; t < n; t += stride) p[t] = v;
 }
 
-__global__ void k_masked_copy(const float* __restrict__ src, const uint8_t* __restrict__ m, float* __restrict__ dst,
-                              size_t n, const int* pred) {
+template <class T>
+__global__ void k_masked_copy(const T* __restrict__ src, const uint8_t* __restrict__ m, T* __restrict__ dst, size_t n,
+                              const int* pred) {
   if (pred_off(pred)) return;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    dst[i] = m[i] ? src[i] : 0.f;
+    dst[i] = m[i] ? src[i] : (T)0;
 }
 
 __global__ void k_fill_f32(float* p, size_t n, float v) {
@@ -84,13 +86,14 @@ __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
 }
 
 // dst (n x n) = src window [r0, r0+n) x [c0, c0+n) of a (ld-wide) plane, optional multiply by mask window
-__global__ void k_window(const float* __restrict__ src, int ld, int r0, int c0, int n, const float* __restrict__ mul,
-                         float* __restrict__ dst) {
+template <class T, class TIN>
+__global__ void k_window(const TIN* __restrict__ src, int ld, int r0, int c0, int n, const T* __restrict__ mul,
+                         T* __restrict__ dst) {
   const size_t tot = (size_t)n * n;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
     const int r = (int)(i / n), c = (int)(i % n);
     const size_t s = (size_t)(r0 + r) * ld + (c0 + c);
-    float v = src[s];
+    T v = (T)src[s];
     if (mul) v *= mul[s];
     dst[i] = v;
   }
@@ -107,40 +110,63 @@ __device__ __forceinline__ bool in_noise_mask(int r, int c, Rect inc, Rect exc) 
   return in && !ex;
 }
 
-// per-plane select state
-struct SelState {
-  uint32_t prefix;   // fixed high bits of the target key
+// Sort keys of |w|: the bits of a non-negative float (or double) order like an unsigned integer.
+template <class T>
+struct MadKey;
+template <>
+struct MadKey<float> {
+  using K = uint32_t;
+  __device__ static K key(float v) { return __float_as_uint(v) & 0x7fffffffu; }
+  __device__ static float val(K k) { return __uint_as_float(k); }
+};
+template <>
+struct MadKey<double> {
+  using K = unsigned long long;
+  __device__ static K key(double v) { return (K)__double_as_longlong(v) & 0x7fffffffffffffffull; }
+  __device__ static double val(K k) { return __longlong_as_double((long long)k); }
+};
+
+// per-plane select state (32 bytes for either key width)
+template <class K>
+struct alignas(16) SelStateT {
+  K prefix;          // fixed high bits of the target key
+  K hi_key;          // min key above the low median (for even counts)
   uint32_t rank;     // remaining rank of the low median inside the prefix bucket
   uint32_t eq;       // count of keys equal to the final low median
   uint32_t first;    // rank of the first key equal to the low median
-  uint32_t hi_key;   // min key above the low median (for even counts)
   uint32_t n;        // number of keys
-  uint32_t pad0, pad1;
 };
+constexpr size_t kSelBytes = 32;
+static_assert(sizeof(SelStateT<uint32_t>) <= kSelBytes && sizeof(SelStateT<unsigned long long>) <= kSelBytes,
+              "select state exceeds its slot");
 
 constexpr int kBins = 2048;
 constexpr int kMaxPlanes = 16;  // plane stacks handled per call (scale counts up to 16)
 
-template <bool MASKED>
-__global__ __launch_bounds__(kT) void k_mad_hist(const float* __restrict__ planes, size_t plane, int w, Rect inc,
+template <class T, bool MASKED>
+__global__ __launch_bounds__(kT) void k_mad_hist(const T* __restrict__ planes, size_t plane, int w, Rect inc,
                                                  Rect exc, int shift, int nbits, int prefix_shift,
-                                                 const SelState* __restrict__ st, uint32_t* __restrict__ hist,
-                                                 const int* pred) {
+                                                 const SelStateT<typename MadKey<T>::K>* __restrict__ st,
+                                                 uint32_t* __restrict__ hist, const int* pred) {
+  using K = typename MadKey<T>::K;
   if (pred_off(pred)) return;
   __shared__ uint32_t h[kBins];
   const int s = blockIdx.y;
   for (int i = threadIdx.x; i < kBins; i += kT) h[i] = 0;
   __syncthreads();
-  const float* x = planes + (size_t)s * plane;
-  const uint32_t pre = st[s].prefix;
-  const uint32_t dmask = (1u << nbits) - 1u;
-  auto add = [&](float v) {
-    const uint32_t key = __float_as_uint(v) & 0x7fffffffu;
-    if (prefix_shift < 32 && (key >> prefix_shift) != (pre >> prefix_shift)) return;
-    atomicAdd(&h[(key >> shift) & dmask], 1u);
+  const T* x = planes + (size_t)s * plane;
+  const K pre = st[s].prefix;
+  const K dmask = ((K)1 << nbits) - 1;
+  auto add = [&](T v) {
+    const K key = MadKey<T>::key(v);
+    if (prefix_shift < (int)(8 * sizeof(K)) && (key >> prefix_shift) != (pre >> prefix_shift)) return;
+    atomicAdd(&h[(uint32_t)((key >> shift) & dmask)], 1u);
   };
   const size_t stride = (size_t)gridDim.x * kT;
-  if (!MASKED && (reinterpret_cast<uintptr_t>(x) & 15) == 0) {
+  bool vec = false;
+  if constexpr (!MASKED && sizeof(T) == 4) vec = (reinterpret_cast<uintptr_t>(x) & 15) == 0;
+  if constexpr (sizeof(T) == 4) {
+    if (vec) {
     // eight keys per thread and iteration, loaded as two float4 before any histogram update, so a
     // thread has two loads in flight instead of one per atomic (the counts do not depend on order)
     const float4* x4 = reinterpret_cast<const float4*>(x);
@@ -155,7 +181,9 @@ __global__ __launch_bounds__(kT) void k_mad_hist(const float* __restrict__ plane
       add(a.x), add(a.y), add(a.z), add(a.w);
     }
     for (size_t j = 4 * n4 + blockIdx.x * (size_t)kT + threadIdx.x; j < plane; j += stride) add(x[j]);
-  } else {
+    }
+  }
+  if (!vec) {
     for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < plane; i += stride) {
       if (MASKED) {
         const int r = (int)(i / w), c = (int)(i % w);
@@ -171,7 +199,8 @@ __global__ __launch_bounds__(kT) void k_mad_hist(const float* __restrict__ plane
 }
 
 // one block per plane: locate the bucket holding the target rank, extend the prefix, clear the histogram
-__global__ __launch_bounds__(kT) void k_mad_select(SelState* st, uint32_t* hist, int shift, int nbits, int last,
+template <class K>
+__global__ __launch_bounds__(kT) void k_mad_select(SelStateT<K>* st, uint32_t* hist, int shift, int nbits, int last,
                                                    const int* pred) {
   if (pred_off(pred)) return;
   const int s = blockIdx.x;
@@ -180,7 +209,7 @@ __global__ __launch_bounds__(kT) void k_mad_select(SelState* st, uint32_t* hist,
   __shared__ uint32_t part[kT];
   const int nb = 1 << nbits;
   const int per = kBins / kT;  // 8 bins per thread
-  const SelState S0 = st[s];   // read by every thread before the barrier; one thread writes it after
+  const SelStateT<K> S0 = st[s];  // read by every thread before the barrier; one thread writes it after
   uint32_t local = 0;
   for (int k = 0; k < per; ++k) {
     const int b = threadIdx.x * per + k;
@@ -207,7 +236,7 @@ __global__ __launch_bounds__(kT) void k_mad_select(SelState* st, uint32_t* hist,
   // last thread also takes a rank beyond the total, as the serial scan did)
   const bool mine = (S0.rank >= exc && S0.rank < inc) || (threadIdx.x == kT - 1 && S0.rank >= inc);
   if (mine) {
-    SelState S = S0;
+    SelStateT<K> S = S0;
     uint32_t run = exc;
     const int t = threadIdx.x;
     int b = t * per;
@@ -216,38 +245,45 @@ __global__ __launch_bounds__(kT) void k_mad_select(SelState* st, uint32_t* hist,
       run += cnt[b];
     }
     S.rank -= run;
-    S.prefix |= (uint32_t)b << shift;
+    S.prefix |= (K)b << shift;
     if (last) {
       S.eq = cnt[b];
-      S.hi_key = 0xffffffffu;
+      S.hi_key = ~(K)0;
     }
     st[s] = S;
   }
 }
 
 // min key strictly above the low median (even counts whose two middles differ)
-template <bool MASKED>
-__global__ __launch_bounds__(kT) void k_mad_above(const float* __restrict__ planes, size_t plane, int w, Rect inc,
-                                                  Rect exc, SelState* st, const int* pred) {
+template <class T, bool MASKED>
+__global__ __launch_bounds__(kT) void k_mad_above(const T* __restrict__ planes, size_t plane, int w, Rect inc,
+                                                  Rect exc, SelStateT<typename MadKey<T>::K>* st, const int* pred) {
+  using K = typename MadKey<T>::K;
   if (pred_off(pred)) return;
   const int s = blockIdx.y;
-  const float* x = planes + (size_t)s * plane;
-  const uint32_t lo = st[s].prefix;
-  uint32_t best = 0xffffffffu;
-  auto take = [&](float v) {
-    const uint32_t key = __float_as_uint(v) & 0x7fffffffu;
+  const T* x = planes + (size_t)s * plane;
+  const K lo = st[s].prefix;
+  const K none = ~(K)0;
+  K best = none;
+  auto take = [&](T v) {
+    const K key = MadKey<T>::key(v);
     if (key > lo && key < best) best = key;
   };
   const size_t stride = (size_t)gridDim.x * kT;
-  if (!MASKED && (reinterpret_cast<uintptr_t>(x) & 15) == 0) {
-    const float4* x4 = reinterpret_cast<const float4*>(x);
-    const size_t n4 = plane >> 2;
-    for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n4; i += stride) {
-      const float4 a = x4[i];
-      take(a.x), take(a.y), take(a.z), take(a.w);
+  bool vec = false;
+  if constexpr (!MASKED && sizeof(T) == 4) vec = (reinterpret_cast<uintptr_t>(x) & 15) == 0;
+  if constexpr (sizeof(T) == 4) {
+    if (vec) {
+      const float4* x4 = reinterpret_cast<const float4*>(x);
+      const size_t n4 = plane >> 2;
+      for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n4; i += stride) {
+        const float4 a = x4[i];
+        take(a.x), take(a.y), take(a.z), take(a.w);
+      }
+      for (size_t j = 4 * n4 + blockIdx.x * (size_t)kT + threadIdx.x; j < plane; j += stride) take(x[j]);
     }
-    for (size_t j = 4 * n4 + blockIdx.x * (size_t)kT + threadIdx.x; j < plane; j += stride) take(x[j]);
-  } else {
+  }
+  if (!vec) {
     for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < plane; i += stride) {
       if (MASKED) {
         const int r = (int)(i / w), c = (int)(i % w);
@@ -258,108 +294,121 @@ __global__ __launch_bounds__(kT) void k_mad_above(const float* __restrict__ plan
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
-    const uint32_t v = __shfl_xor(best, o, 64);
+    const K v = __shfl_xor(best, o, 64);
     best = v < best ? v : best;
   }
   // one atomic per block (per-wave atomics on the one address serialise at the L2)
-  __shared__ uint32_t wmin[kT / 64];
+  __shared__ K wmin[kT / 64];
   if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = best;
   __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 1; k < kT / 64; ++k) best = wmin[k] < best ? wmin[k] : best;
-    if (best != 0xffffffffu) atomicMin(&st[s].hi_key, best);
+    if (best != none) atomicMin(&st[s].hi_key, best);
   }
 }
 
-__global__ void k_mad_init(SelState* st, int S, uint32_t n) {
+template <class K>
+__global__ void k_mad_init(SelStateT<K>* st, int S, uint32_t n) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S) return;
-  SelState z{};
+  SelStateT<K> z{};
   z.n = n;
   z.rank = (n - 1) / 2;
   z.prefix = 0;
   st[s] = z;
 }
 
-__global__ void k_mad_finish(const SelState* st, int S, double* thr, const int* pred) {
+template <class T>
+__global__ void k_mad_finish(const SelStateT<typename MadKey<T>::K>* st, int S, double* thr, const int* pred) {
   if (pred_off(pred)) return;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S) return;
-  const SelState z = st[s];
+  const SelStateT<typename MadKey<T>::K> z = st[s];
   const uint32_t klo = (z.n - 1) / 2;
-  const float vlo = __uint_as_float(z.prefix);
+  const T vlo = MadKey<T>::val(z.prefix);
   double med = (double)vlo;
   if ((z.n & 1u) == 0) {
     // rank of the last key equal to vlo = first + eq - 1, first = klo - remaining rank
     const uint32_t first = klo - z.rank;
     const uint32_t lastr = first + z.eq - 1;
-    const float vhi = (klo + 1 <= lastr) ? vlo : __uint_as_float(z.hi_key);
+    const T vhi = (klo + 1 <= lastr) ? vlo : MadKey<T>::val(z.hi_key);
     med = ((double)vlo + (double)vhi) * 0.5;
   }
   thr[s] = med / 0.6745;
 }
 
 // ----------------------------------------------------------------------------- threshold + maxima
-__global__ __launch_bounds__(kT) void k_threshold(const float* __restrict__ in, float* __restrict__ out, size_t plane,
-                                                  int w, const double* __restrict__ thr, double sigma,
-                                                  const int* __restrict__ border, float* __restrict__ pmax,
+template <class T>
+__global__ __launch_bounds__(kT) void k_threshold(const T* __restrict__ in, T* __restrict__ out, size_t plane, int w,
+                                                  const double* __restrict__ thr, double sigma,
+                                                  const int* __restrict__ border, T* __restrict__ pmax,
                                                   const int* pred) {
   if (pred_off(pred)) return;
   const int s = blockIdx.y;
   const int h = (int)(plane / w);
   const double lim = sigma * thr[s];
   const int b = border ? border[s] : 0;
-  const float* x = in + (size_t)s * plane;
-  float* y = out + (size_t)s * plane;
-  float m = -INFINITY;
+  const T* x = in + (size_t)s * plane;
+  T* y = out + (size_t)s * plane;
+  T m = -INFINITY;
   for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < plane; i += (size_t)gridDim.x * kT) {
-    const float v = x[i];
-    float o = ((double)fabsf(v) > lim) ? v : 0.f;
+    const T v = x[i];
+    T o = ((double)fabs(v) > lim) ? v : (T)0;
     if (border) {
       const int r = (int)(i / w), c = (int)(i % w);
-      if (r < b || r >= h - b || c < b || c >= w - b) o = 0.f;
+      if (r < b || r >= h - b || c < b || c >= w - b) o = 0;
     }
     y[i] = o;
-    m = fmaxf(m, o);
+    m = o > m ? o : m;
   }
-  m = block_max_f32(m);
+  m = block_max_t<T>(m);
   if (threadIdx.x == 0) pmax[(size_t)s * gridDim.x + blockIdx.x] = m;
 }
 
 // per-plane max (or max |w|) of a plane stack: partials[s * gridDim.x + block]
-__global__ __launch_bounds__(kT) void k_plane_max(const float* __restrict__ in, size_t plane, int use_abs,
-                                                  float* __restrict__ pmax, const int* pred) {
+template <class T>
+__global__ __launch_bounds__(kT) void k_plane_max(const T* __restrict__ in, size_t plane, int use_abs,
+                                                  T* __restrict__ pmax, const int* pred) {
   if (pred_off(pred)) return;
   const int s = blockIdx.y;
-  const float* x = in + (size_t)s * plane;
-  float m = -INFINITY;
-  auto take = [&](float v) { m = fmaxf(m, use_abs ? fabsf(v) : v); };
+  const T* x = in + (size_t)s * plane;
+  T m = -INFINITY;
+  auto take = [&](T v) {
+    const T a = use_abs ? (T)fabs(v) : v;
+    m = a > m ? a : m;
+  };
   const size_t stride = (size_t)gridDim.x * kT;
   size_t tail = 0;
-  if ((reinterpret_cast<uintptr_t>(x) & 15) == 0) {  // float4 loads; the maximum does not depend on order
-    const float4* x4 = reinterpret_cast<const float4*>(x);
-    const size_t n4 = plane >> 2;
-    for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n4; i += stride) {
-      const float4 a = x4[i];
-      take(a.x), take(a.y), take(a.z), take(a.w);
+  if constexpr (sizeof(T) == 4) {
+    if ((reinterpret_cast<uintptr_t>(x) & 15) == 0) {  // float4 loads; the maximum does not depend on order
+      const float4* x4 = reinterpret_cast<const float4*>(x);
+      const size_t n4 = plane >> 2;
+      for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n4; i += stride) {
+        const float4 a = x4[i];
+        take(a.x), take(a.y), take(a.z), take(a.w);
+      }
+      tail = 4 * n4;
     }
-    tail = 4 * n4;
   }
   for (size_t i = tail + blockIdx.x * (size_t)kT + threadIdx.x; i < plane; i += stride) take(x[i]);
-  m = block_max_f32(m);
+  m = block_max_t<T>(m);
   if (threadIdx.x == 0) pmax[(size_t)s * gridDim.x + blockIdx.x] = m;
 }
 
 // per-plane reduce of partial maxima; out_max[s] (float) and optionally nsm[s] = max / energy[s]
 // (one block per plane; max is exact in any order)
-__global__ __launch_bounds__(kT) void k_max_final(const float* pmax, int nb, float* out_max, const double* energy,
+template <class T>
+__global__ __launch_bounds__(kT) void k_max_final(const T* pmax, int nb, T* out_max, const double* energy,
                                                   double* nsm, const int* pred) {
   if (pred_off(pred)) return;
   const int s = blockIdx.x;
-  float m = -INFINITY;
-  for (int b = threadIdx.x; b < nb; b += kT) m = fmaxf(m, pmax[(size_t)s * nb + b]);
-  m = block_max_f32(m);
+  T m = -INFINITY;
+  for (int b = threadIdx.x; b < nb; b += kT) {
+    const T v = pmax[(size_t)s * nb + b];
+    m = v > m ? v : m;
+  }
+  m = block_max_t<T>(m);
   if (threadIdx.x == 0) {
     if (out_max) out_max[s] = m;
     if (nsm) nsm[s] = (double)m / energy[s];
@@ -424,7 +473,8 @@ __device__ __forceinline__ void lds_union(int* L, int a, int b) {
   }
 }
 
-__global__ __launch_bounds__(kT) void k_ccl_tile(const float* __restrict__ w, int* __restrict__ lab, int h, int wd,
+template <class V>
+__global__ __launch_bounds__(kT) void k_ccl_tile(const V* __restrict__ w, int* __restrict__ lab, int h, int wd,
                                                  const int* pred) {
   if (pred_off(pred)) return;
   constexpr int T = kCclTile, TT = kCclTile * kCclTile;
@@ -433,11 +483,11 @@ __global__ __launch_bounds__(kT) void k_ccl_tile(const float* __restrict__ w, in
   const int tiles_x = (wd + T - 1) / T;
   const int r0 = (blockIdx.x / tiles_x) * T, c0 = (blockIdx.x % tiles_x) * T;
   const size_t plane = (size_t)h * wd;
-  const float* x = w + (size_t)s * plane;
+  const V* x = w + (size_t)s * plane;
   int* l = lab + (size_t)s * plane;
   for (int i = threadIdx.x; i < TT; i += kT) {
     const int r = r0 + i / T, c = c0 + i % T;
-    L[i] = (r < h && c < wd && x[(size_t)r * wd + c] != 0.f) ? i : -1;
+    L[i] = (r < h && c < wd && x[(size_t)r * wd + c] != (V)0) ? i : -1;
   }
   __syncthreads();
   for (int i = threadIdx.x; i < TT; i += kT) {
@@ -521,18 +571,19 @@ __global__ __launch_bounds__(kT) void k_ccl_compress(int* __restrict__ lab, size
 }
 
 // mark roots of objects holding a seed pixel (value >= tol*max, inside the coarser kept mask)
-__global__ __launch_bounds__(kT) void k_keep_mark(const float* __restrict__ w, const int* __restrict__ lab,
-                                                  const uint8_t* __restrict__ coarse_mask, const float* __restrict__ smax,
+template <class T>
+__global__ __launch_bounds__(kT) void k_keep_mark(const T* __restrict__ w, const int* __restrict__ lab,
+                                                  const uint8_t* __restrict__ coarse_mask, const T* __restrict__ smax,
                                                   int s, double tol, int neg_comp, size_t plane,
                                                   uint8_t* __restrict__ keep, const int* pred) {
   if (pred_off(pred)) return;
   const double lim = tol * (double)smax[s];
-  const float* x = w + (size_t)s * plane;
+  const T* x = w + (size_t)s * plane;
   const int* l = lab + (size_t)s * plane;
   for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < plane; i += (size_t)gridDim.x * kT) {
     const int root = l[i];
     if (root < 0) continue;
-    const float v = neg_comp ? fabsf(x[i]) : x[i];
+    const T v = neg_comp ? (T)fabs(x[i]) : x[i];
     if (!((double)v >= lim)) continue;
     if (coarse_mask && coarse_mask[i] == 0) continue;
     keep[root] = 1;
@@ -540,20 +591,21 @@ __global__ __launch_bounds__(kT) void k_keep_mark(const float* __restrict__ w, c
 }
 
 // mask = object kept; extracted = mask * w; keep flags are consumed and cleared
-__global__ __launch_bounds__(kT) void k_keep_apply(const float* __restrict__ w, const int* __restrict__ lab, int s,
+template <class T>
+__global__ __launch_bounds__(kT) void k_keep_apply(const T* __restrict__ w, const int* __restrict__ lab, int s,
                                                    size_t plane, const uint8_t* __restrict__ keep,
-                                                   uint8_t* __restrict__ mask, float* __restrict__ extracted,
+                                                   uint8_t* __restrict__ mask, T* __restrict__ extracted,
                                                    const int* pred) {
   if (pred_off(pred)) return;
-  const float* x = w + (size_t)s * plane;
+  const T* x = w + (size_t)s * plane;
   const int* l = lab + (size_t)s * plane;
   uint8_t* m = mask + (size_t)s * plane;
-  float* e = extracted ? extracted + (size_t)s * plane : nullptr;
+  T* e = extracted ? extracted + (size_t)s * plane : nullptr;
   for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < plane; i += (size_t)gridDim.x * kT) {
     const int root = l[i];
     const uint8_t k = (root >= 0) ? keep[root] : (uint8_t)0;
     m[i] = k;
-    if (e) e[i] = k ? x[i] : 0.f;
+    if (e) e[i] = k ? x[i] : (T)0;
   }
 }
 
@@ -620,7 +672,8 @@ __global__ __launch_bounds__(kT) void k_label_out(const int* __restrict__ lab, c
 
 // ----------------------------------------------------------------------------- reductions & CG vectors
 // partial[b] = sum over the block's grid-stride range of a[i]*b[i] (b == null: a[i]^2)
-__global__ __launch_bounds__(kT) void k_dot(const float* __restrict__ a, const float* __restrict__ b, size_t n,
+template <class T>
+__global__ __launch_bounds__(kT) void k_dot(const T* __restrict__ a, const T* __restrict__ b, size_t n,
                                             double* __restrict__ partial, const int* pred) {
   if (pred_off(pred)) return;
   double acc = 0.0;
@@ -633,11 +686,12 @@ __global__ __launch_bounds__(kT) void k_dot(const float* __restrict__ a, const f
 }
 
 // per-plane sums of squares of a plane stack: partial[s * gridDim.x + block]
-__global__ __launch_bounds__(kT) void k_plane_sumsq(const float* __restrict__ a, size_t plane,
+template <class T>
+__global__ __launch_bounds__(kT) void k_plane_sumsq(const T* __restrict__ a, size_t plane,
                                                     double* __restrict__ partial, const int* pred) {
   if (pred_off(pred)) return;
   const int s = blockIdx.y;
-  const float* x = a + (size_t)s * plane;
+  const T* x = a + (size_t)s * plane;
   double acc = 0.0;
   for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < plane; i += (size_t)gridDim.x * kT) {
     const double v = x[i];
@@ -648,7 +702,8 @@ __global__ __launch_bounds__(kT) void k_plane_sumsq(const float* __restrict__ a,
 }
 
 // sum of (x - mean)^2 where mean = *mean_sum / count
-__global__ __launch_bounds__(kT) void k_dev2(const float* __restrict__ a, size_t n, const double* __restrict__ sum,
+template <class T>
+__global__ __launch_bounds__(kT) void k_dev2(const T* __restrict__ a, size_t n, const double* __restrict__ sum,
                                              double* __restrict__ partial) {
   const double mean = *sum / (double)n;
   double acc = 0.0;
@@ -660,7 +715,8 @@ __global__ __launch_bounds__(kT) void k_dev2(const float* __restrict__ a, size_t
   if (threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
 
-__global__ __launch_bounds__(kT) void k_sum(const float* __restrict__ a, size_t n, double* __restrict__ partial) {
+template <class T>
+__global__ __launch_bounds__(kT) void k_sum(const T* __restrict__ a, size_t n, double* __restrict__ partial) {
   double acc = 0.0;
   for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) acc += (double)a[i];
   acc = block_sum_f64(acc);
@@ -722,13 +778,25 @@ void copy_plane(const float* src, float* dst, size_t n, const int* pred, hipStre
   if (aligned16(src, dst))
     hipLaunchKernelGGL(k_copy, dim3(vec_grid(n)), dim3(kT), 0, st, src, dst, n, pred);
   else
-    hipLaunchKernelGGL(k_copy1, dim3(ew_grid(n)), dim3(kT), 0, st, src, dst, n, pred);
+    hipLaunchKernelGGL(k_copy1<float>, dim3(ew_grid(n)), dim3(kT), 0, st, src, dst, n, pred);
   PMS_LAUNCH_CHECK();
 }
 
-void masked_copy(const float* src, const uint8_t* mask, float* dst, size_t n, const int* pred, hipStream_t st) {
-  hipLaunchKernelGGL(k_masked_copy, dim3(ew_grid(n)), dim3(kT), 0, st, src, mask, dst, n, pred);
+// a double plane is copied as twice as many floats (bit copy)
+void copy_plane(const double* src, double* dst, size_t n, const int* pred, hipStream_t st) {
+  copy_plane(reinterpret_cast<const float*>(src), reinterpret_cast<float*>(dst), 2 * n, pred, st);
+}
+
+template <class T>
+static void masked_copy_t(const T* src, const uint8_t* mask, T* dst, size_t n, const int* pred, hipStream_t st) {
+  hipLaunchKernelGGL(k_masked_copy<T>, dim3(ew_grid(n)), dim3(kT), 0, st, src, mask, dst, n, pred);
   PMS_LAUNCH_CHECK();
+}
+void masked_copy(const float* src, const uint8_t* mask, float* dst, size_t n, const int* pred, hipStream_t st) {
+  masked_copy_t(src, mask, dst, n, pred, st);
+}
+void masked_copy(const double* src, const uint8_t* mask, double* dst, size_t n, const int* pred, hipStream_t st) {
+  masked_copy_t(src, mask, dst, n, pred, st);
 }
 
 void fill_f32(float* p, size_t n, float v, hipStream_t st) {
@@ -747,9 +815,23 @@ void fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t st) {
   PMS_LAUNCH_CHECK();
 }
 
-void window_copy(const float* src, int ld, int r0, int c0, int n, const float* mul, float* dst, hipStream_t st) {
-  hipLaunchKernelGGL(k_window, dim3(ew_grid((size_t)n * n)), dim3(kT), 0, st, src, ld, r0, c0, n, mul, dst);
+void fill_zero(float* p, size_t n, hipStream_t st) { fill_u32(reinterpret_cast<uint32_t*>(p), n, 0u, st); }
+void fill_zero(double* p, size_t n, hipStream_t st) { fill_u32(reinterpret_cast<uint32_t*>(p), 2 * n, 0u, st); }
+
+template <class T, class TIN>
+static void window_copy_t(const TIN* src, int ld, int r0, int c0, int n, const T* mul, T* dst, hipStream_t st) {
+  hipLaunchKernelGGL((k_window<T, TIN>), dim3(ew_grid((size_t)n * n)), dim3(kT), 0, st, src, ld, r0, c0, n, mul,
+                     dst);
   PMS_LAUNCH_CHECK();
+}
+void window_copy(const float* src, int ld, int r0, int c0, int n, const float* mul, float* dst, hipStream_t st) {
+  window_copy_t(src, ld, r0, c0, n, mul, dst, st);
+}
+void window_copy(const float* src, int ld, int r0, int c0, int n, const double* mul, double* dst, hipStream_t st) {
+  window_copy_t(src, ld, r0, c0, n, mul, dst, st);
+}
+void window_copy(const double* src, int ld, int r0, int c0, int n, const double* mul, double* dst, hipStream_t st) {
+  window_copy_t(src, ld, r0, c0, n, mul, dst, st);
 }
 
 // Python slice(start, stop).indices(n) for step 1, as [lo, hi) with hi >= lo
@@ -779,7 +861,9 @@ NoiseMask noise_mask(int h, int w, int edge_excl, int int_excl) {
     py_slice(edge_excl, -edge_excl, w, &m.inc_c0, &m.inc_c1);
   }
   if (int_excl != 0) {
-    const int mid = w / 2;
+    // iuwt_toolbox.py:29,40: mid = in1.shape[1]/2 is the row count of the (scales, rows, cols) stack,
+    // and the same mid bounds both axes
+    const int mid = h / 2;
     py_slice(mid - int_excl, mid + int_excl, h, &m.exc_r0, &m.exc_r1);
     py_slice(mid - int_excl, mid + int_excl, w, &m.exc_c0, &m.exc_c1);
   }
@@ -791,79 +875,129 @@ NoiseMask noise_mask(int h, int w, int edge_excl, int int_excl) {
   return m;
 }
 
-// workspace layout is independent of the plane count of a call: [SelState x 16][histograms 16 x 2048]
-size_t mad_work_bytes(int) { return (size_t)kMaxPlanes * (sizeof(SelState) + kBins * sizeof(uint32_t)); }
+// workspace layout is independent of the plane count and element type of a call:
+// [select state: 16 slots of 32 bytes][histograms 16 x 2048]
+size_t mad_work_bytes(int) { return (size_t)kMaxPlanes * (kSelBytes + kBins * sizeof(uint32_t)); }
 
 void mad_work_init(void* work, int, hipStream_t st) {
-  uint32_t* hist = (uint32_t*)((char*)work + (size_t)kMaxPlanes * sizeof(SelState));
+  uint32_t* hist = (uint32_t*)((char*)work + (size_t)kMaxPlanes * kSelBytes);
   fill_u32(hist, (size_t)kMaxPlanes * kBins, 0u, st);
 }
 
-void mad_thresholds(const float* planes, int S, int h, int w, const NoiseMask& nm, double* thr, void* work,
-                    const int* pred, hipStream_t st) {
+// Exact median per plane: radix select over the key bits in digits of at most 11 bits, one
+// histogram pass and one select step per digit, then the successor of the low median for even
+// counts.
+template <class T>
+static void mad_thresholds_t(const T* planes, int S, int h, int w, const NoiseMask& nm, double* thr, void* work,
+                             const int* pred, hipStream_t st) {
+  using K = typename MadKey<T>::K;
   PMS_REQUIRE(nm.count > 0, "noise mask selects no pixels");
   PMS_REQUIRE(nm.count < (1L << 32), "plane too large for the median");
   PMS_REQUIRE(S >= 1 && S <= kMaxPlanes, "the median handles 1..16 planes per call");
-  SelState* sst = (SelState*)work;
-  uint32_t* hist = (uint32_t*)((char*)work + (size_t)kMaxPlanes * sizeof(SelState));
+  SelStateT<K>* sst = (SelStateT<K>*)work;
+  uint32_t* hist = (uint32_t*)((char*)work + (size_t)kMaxPlanes * kSelBytes);
   const size_t plane = (size_t)h * w;
   Rect inc{nm.inc_r0, nm.inc_r1, nm.inc_c0, nm.inc_c1};
   Rect exc{nm.exc_r0, nm.exc_r1, nm.exc_c0, nm.exc_c1};
-  hipLaunchKernelGGL(k_mad_init, dim3(ceil_div(S, 64)), dim3(64), 0, st, sst, S, (uint32_t)nm.count);
+  hipLaunchKernelGGL(k_mad_init<K>, dim3(ceil_div(S, 64)), dim3(64), 0, st, sst, S, (uint32_t)nm.count);
   PMS_LAUNCH_CHECK();
   const int gx = std::min(ew_grid(plane), 512);
-  const int shifts[3] = {21, 10, 0};
-  const int bits[3] = {11, 11, 10};
-  for (int pass = 0; pass < 3; ++pass) {
-    const int pshift = (pass == 0) ? 32 : shifts[pass - 1];
+  // digits of 11 bits from the top of the key word (its sign bit is always clear); the last digit
+  // takes what is left: 11/11/10 bits for float, 11 x 5 + 9 for double
+  constexpr int kKeyBits = 8 * sizeof(K);
+  constexpr int kPasses = (kKeyBits + 10) / 11;
+  int shifts[kPasses], bits[kPasses];
+  for (int pass = 0, top = kKeyBits; pass < kPasses; ++pass) {
+    bits[pass] = std::min(11, top);
+    top -= bits[pass];
+    shifts[pass] = top;
+  }
+  for (int pass = 0; pass < kPasses; ++pass) {
+    const int pshift = (pass == 0) ? (int)(8 * sizeof(K)) : shifts[pass - 1];
     if (nm.masked)
-      hipLaunchKernelGGL(k_mad_hist<true>, dim3(gx, S), dim3(kT), 0, st, planes, plane, w, inc, exc, shifts[pass],
-                         bits[pass], pshift, (const SelState*)sst, hist, pred);
+      hipLaunchKernelGGL((k_mad_hist<T, true>), dim3(gx, S), dim3(kT), 0, st, planes, plane, w, inc, exc,
+                         shifts[pass], bits[pass], pshift, (const SelStateT<K>*)sst, hist, pred);
     else
-      hipLaunchKernelGGL(k_mad_hist<false>, dim3(gx, S), dim3(kT), 0, st, planes, plane, w, inc, exc, shifts[pass],
-                         bits[pass], pshift, (const SelState*)sst, hist, pred);
+      hipLaunchKernelGGL((k_mad_hist<T, false>), dim3(gx, S), dim3(kT), 0, st, planes, plane, w, inc, exc,
+                         shifts[pass], bits[pass], pshift, (const SelStateT<K>*)sst, hist, pred);
     PMS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_mad_select, dim3(S), dim3(kT), 0, st, sst, hist, shifts[pass], bits[pass], pass == 2, pred);
+    hipLaunchKernelGGL(k_mad_select<K>, dim3(S), dim3(kT), 0, st, sst, hist, shifts[pass], bits[pass],
+                       pass == kPasses - 1, pred);
     PMS_LAUNCH_CHECK();
   }
   if ((nm.count & 1) == 0) {
     if (nm.masked)
-      hipLaunchKernelGGL(k_mad_above<true>, dim3(gx, S), dim3(kT), 0, st, planes, plane, w, inc, exc, sst, pred);
+      hipLaunchKernelGGL((k_mad_above<T, true>), dim3(gx, S), dim3(kT), 0, st, planes, plane, w, inc, exc, sst, pred);
     else
-      hipLaunchKernelGGL(k_mad_above<false>, dim3(gx, S), dim3(kT), 0, st, planes, plane, w, inc, exc, sst, pred);
+      hipLaunchKernelGGL((k_mad_above<T, false>), dim3(gx, S), dim3(kT), 0, st, planes, plane, w, inc, exc, sst,
+                         pred);
     PMS_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL(k_mad_finish, dim3(ceil_div(S, 64)), dim3(64), 0, st, (const SelState*)sst, S, thr, pred);
+  hipLaunchKernelGGL(k_mad_finish<T>, dim3(ceil_div(S, 64)), dim3(64), 0, st, (const SelStateT<K>*)sst, S, thr, pred);
   PMS_LAUNCH_CHECK();
+}
+void mad_thresholds(const float* planes, int S, int h, int w, const NoiseMask& nm, double* thr, void* work,
+                    const int* pred, hipStream_t st) {
+  mad_thresholds_t(planes, S, h, w, nm, thr, work, pred, st);
+}
+void mad_thresholds(const double* planes, int S, int h, int w, const NoiseMask& nm, double* thr, void* work,
+                    const int* pred, hipStream_t st) {
+  mad_thresholds_t(planes, S, h, w, nm, thr, work, pred, st);
 }
 
 int threshold_grid(size_t plane) { return std::min(ew_grid(plane), 256); }
 
-void apply_threshold(const float* in, float* out, int S, int h, int w, const double* thr, double sigma,
-                     const int* border, float* pmax, const int* pred, hipStream_t st) {
+template <class T>
+static void apply_threshold_t(const T* in, T* out, int S, int h, int w, const double* thr, double sigma,
+                              const int* border, T* pmax, const int* pred, hipStream_t st) {
   const size_t plane = (size_t)h * w;
   const int gx = threshold_grid(plane);
-  hipLaunchKernelGGL(k_threshold, dim3(gx, S), dim3(kT), 0, st, in, out, plane, w, thr, sigma, border, pmax, pred);
+  hipLaunchKernelGGL(k_threshold<T>, dim3(gx, S), dim3(kT), 0, st, in, out, plane, w, thr, sigma, border, pmax, pred);
   PMS_LAUNCH_CHECK();
 }
+void apply_threshold(const float* in, float* out, int S, int h, int w, const double* thr, double sigma,
+                     const int* border, float* pmax, const int* pred, hipStream_t st) {
+  apply_threshold_t(in, out, S, h, w, thr, sigma, border, pmax, pred, st);
+}
+void apply_threshold(const double* in, double* out, int S, int h, int w, const double* thr, double sigma,
+                     const int* border, double* pmax, const int* pred, hipStream_t st) {
+  apply_threshold_t(in, out, S, h, w, thr, sigma, border, pmax, pred, st);
+}
 
-void plane_max(const float* in, int S, size_t plane, int use_abs, float* pmax, const int* pred, hipStream_t st) {
+template <class T>
+static void plane_max_t(const T* in, int S, size_t plane, int use_abs, T* pmax, const int* pred, hipStream_t st) {
   const int gx = threshold_grid(plane);
-  hipLaunchKernelGGL(k_plane_max, dim3(gx, S), dim3(kT), 0, st, in, plane, use_abs, pmax, pred);
+  hipLaunchKernelGGL(k_plane_max<T>, dim3(gx, S), dim3(kT), 0, st, in, plane, use_abs, pmax, pred);
   PMS_LAUNCH_CHECK();
 }
+void plane_max(const float* in, int S, size_t plane, int use_abs, float* pmax, const int* pred, hipStream_t st) {
+  plane_max_t(in, S, plane, use_abs, pmax, pred, st);
+}
+void plane_max(const double* in, int S, size_t plane, int use_abs, double* pmax, const int* pred, hipStream_t st) {
+  plane_max_t(in, S, plane, use_abs, pmax, pred, st);
+}
 
+template <class T>
+static void max_final_t(const T* pmax, int nb, int S, T* out_max, const double* energy, double* nsm, const int* pred,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_max_final<T>, dim3(S), dim3(kT), 0, st, pmax, nb, out_max, energy, nsm, pred);
+  PMS_LAUNCH_CHECK();
+}
 void max_final(const float* pmax, int nb, int S, float* out_max, const double* energy, double* nsm, const int* pred,
                hipStream_t st) {
-  hipLaunchKernelGGL(k_max_final, dim3(S), dim3(kT), 0, st, pmax, nb, out_max, energy, nsm, pred);
-  PMS_LAUNCH_CHECK();
+  max_final_t(pmax, nb, S, out_max, energy, nsm, pred, st);
+}
+void max_final(const double* pmax, int nb, int S, double* out_max, const double* energy, double* nsm,
+               const int* pred, hipStream_t st) {
+  max_final_t(pmax, nb, S, out_max, energy, nsm, pred, st);
 }
 
-// workspace layout: [keep flags: one plane of bytes][maxima: 64 floats][partial maxima: 16 x 2048 floats]
-// [labels: S planes of int]; everything before the labels sits at fixed offsets.
+// workspace layout: [keep flags: one plane of bytes][maxima: 64 values][partial maxima: 16 x 2048 values]
+// [labels: S planes of int]; everything before the labels sits at fixed offsets (values sized for
+// double, so one layout serves both element types).
 static size_t keep_bytes(size_t plane) { return ((plane + 255) / 256) * 256; }
 static size_t ext_labels_offset(size_t plane) {
-  return keep_bytes(plane) + 64 * sizeof(float) + (size_t)kMaxPlanes * 2048 * sizeof(float);
+  return keep_bytes(plane) + 64 * sizeof(double) + (size_t)kMaxPlanes * 2048 * sizeof(double);
 }
 
 size_t extract_work_bytes(int S, int h, int w) {
@@ -877,24 +1011,25 @@ void extract_work_init(void* work, int, int h, int w, hipStream_t st) {
   PMS_LAUNCH_CHECK();
 }
 
-void source_extract(const float* w, int S, int h, int wd, double tol, int neg_comp, uint8_t* mask, float* extracted,
-                    int* labels_out, void* work, const int* pred, hipStream_t st) {
+template <class T>
+static void source_extract_t(const T* w, int S, int h, int wd, double tol, int neg_comp, uint8_t* mask, T* extracted,
+                             int* labels_out, void* work, const int* pred, hipStream_t st) {
   PMS_REQUIRE((size_t)h * wd < (size_t)1 << 31, "plane too large for int labels");
   PMS_REQUIRE(S >= 1 && S <= kMaxPlanes, "source extraction handles 1..16 planes per call");
   const size_t plane = (size_t)h * wd;
   uint8_t* keep = (uint8_t*)work;
-  float* smax = (float*)((char*)work + keep_bytes(plane));
-  float* pmax = smax + 64;
+  T* smax = (T*)((char*)work + keep_bytes(plane));
+  T* pmax = (T*)((char*)work + keep_bytes(plane) + 64 * sizeof(double));
   int* lab = (int*)((char*)work + ext_labels_offset(plane));
   const int gx = threshold_grid(plane);
   // keep flags start cleared whatever the workspace held before
   hipLaunchKernelGGL(k_clear_u8, dim3(ew_grid(plane)), dim3(kT), 0, st, keep, plane, pred);
   PMS_LAUNCH_CHECK();
-  plane_max(w, S, plane, neg_comp, pmax, pred, st);
-  max_final(pmax, gx, S, smax, nullptr, nullptr, pred, st);
+  plane_max_t(w, S, plane, neg_comp, pmax, pred, st);
+  max_final_t(pmax, gx, S, smax, (const double*)nullptr, (double*)nullptr, pred, st);
   const int gl = std::min(ew_grid(plane), 1024);
   const int tiles = ceil_div(h, kCclTile) * ceil_div(wd, kCclTile);
-  hipLaunchKernelGGL(k_ccl_tile, dim3(tiles, S), dim3(kT), 0, st, w, lab, h, wd, pred);
+  hipLaunchKernelGGL(k_ccl_tile<T>, dim3(tiles, S), dim3(kT), 0, st, w, lab, h, wd, pred);
   PMS_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_ccl_border, dim3(std::min(h, 4096), S), dim3(kT), 0, st, lab, h, wd, pred);
   PMS_LAUNCH_CHECK();
@@ -902,17 +1037,17 @@ void source_extract(const float* w, int S, int h, int wd, double tol, int neg_co
   PMS_LAUNCH_CHECK();
   for (int s = S - 1; s >= 0; --s) {
     const uint8_t* coarse = (s == S - 1) ? nullptr : mask + (size_t)(s + 1) * plane;
-    hipLaunchKernelGGL(k_keep_mark, dim3(gl), dim3(kT), 0, st, w, (const int*)lab, coarse, (const float*)smax, s, tol,
+    hipLaunchKernelGGL(k_keep_mark<T>, dim3(gl), dim3(kT), 0, st, w, (const int*)lab, coarse, (const T*)smax, s, tol,
                        neg_comp, plane, keep, pred);
     PMS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_keep_apply, dim3(gl), dim3(kT), 0, st, w, (const int*)lab, s, plane, (const uint8_t*)keep,
-                       mask, extracted, pred);
+    hipLaunchKernelGGL(k_keep_apply<T>, dim3(gl), dim3(kT), 0, st, w, (const int*)lab, s, plane,
+                       (const uint8_t*)keep, mask, extracted, pred);
     PMS_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_clear_u8, dim3(ew_grid(plane)), dim3(kT), 0, st, keep, plane, pred);
     PMS_LAUNCH_CHECK();
   }
   if (labels_out) {
-    // rank roots in raster order (scipy.ndimage.label numbering); uses labels_out as rank scratch first
+    // rank roots in raster order (scipy.ndimage.label numbering)
     const int chunk = 4096;
     const int nb = (int)((plane + chunk - 1) / chunk);
     uint32_t* counts = nullptr;
@@ -933,11 +1068,26 @@ void source_extract(const float* w, int S, int h, int wd, double tol, int neg_co
     PMS_HIP(hipFreeAsync(rank, st));
   }
 }
+void source_extract(const float* w, int S, int h, int wd, double tol, int neg_comp, uint8_t* mask, float* extracted,
+                    int* labels_out, void* work, const int* pred, hipStream_t st) {
+  source_extract_t(w, S, h, wd, tol, neg_comp, mask, extracted, labels_out, work, pred, st);
+}
+void source_extract(const double* w, int S, int h, int wd, double tol, int neg_comp, uint8_t* mask,
+                    double* extracted, int* labels_out, void* work, const int* pred, hipStream_t st) {
+  source_extract_t(w, S, h, wd, tol, neg_comp, mask, extracted, labels_out, work, pred, st);
+}
 
 // ---------------------------------------------------------------- reductions
-void dot_partials(const float* a, const float* b, size_t n, double* partial, const int* pred, hipStream_t st) {
-  hipLaunchKernelGGL(k_dot, dim3(reduce_grid(n)), dim3(kT), 0, st, a, b, n, partial, pred);
+template <class T>
+static void dot_partials_t(const T* a, const T* b, size_t n, double* partial, const int* pred, hipStream_t st) {
+  hipLaunchKernelGGL(k_dot<T>, dim3(reduce_grid(n)), dim3(kT), 0, st, a, b, n, partial, pred);
   PMS_LAUNCH_CHECK();
+}
+void dot_partials(const float* a, const float* b, size_t n, double* partial, const int* pred, hipStream_t st) {
+  dot_partials_t(a, b, n, partial, pred, st);
+}
+void dot_partials(const double* a, const double* b, size_t n, double* partial, const int* pred, hipStream_t st) {
+  dot_partials_t(a, b, n, partial, pred, st);
 }
 
 void sum_partials(const double* partial, long np, int groups, double* out, const int* pred, hipStream_t st) {
@@ -959,19 +1109,33 @@ void sum_partials_batch(const SumJob* jobs, int count, hipStream_t st) {
   }
 }
 
-void plane_sumsq_partials(const float* a, int S, size_t plane, double* partial, const int* pred, hipStream_t st) {
-  hipLaunchKernelGGL(k_plane_sumsq, dim3(reduce_grid(plane), S), dim3(kT), 0, st, a, plane, partial, pred);
+template <class T>
+static void plane_sumsq_partials_t(const T* a, int S, size_t plane, double* partial, const int* pred, hipStream_t st) {
+  hipLaunchKernelGGL(k_plane_sumsq<T>, dim3(reduce_grid(plane), S), dim3(kT), 0, st, a, plane, partial, pred);
   PMS_LAUNCH_CHECK();
 }
+void plane_sumsq_partials(const float* a, int S, size_t plane, double* partial, const int* pred, hipStream_t st) {
+  plane_sumsq_partials_t(a, S, plane, partial, pred, st);
+}
+void plane_sumsq_partials(const double* a, int S, size_t plane, double* partial, const int* pred, hipStream_t st) {
+  plane_sumsq_partials_t(a, S, plane, partial, pred, st);
+}
 
-void mean_std(const float* a, size_t n, double* partial, double* sum_out, double* dev2_out, hipStream_t st) {
+template <class T>
+static void mean_std_t(const T* a, size_t n, double* partial, double* sum_out, double* dev2_out, hipStream_t st) {
   const int g = reduce_grid(n);
-  hipLaunchKernelGGL(k_sum, dim3(g), dim3(kT), 0, st, a, n, partial);
+  hipLaunchKernelGGL(k_sum<T>, dim3(g), dim3(kT), 0, st, a, n, partial);
   PMS_LAUNCH_CHECK();
   sum_partials(partial, g, 1, sum_out, nullptr, st);
-  hipLaunchKernelGGL(k_dev2, dim3(g), dim3(kT), 0, st, a, n, (const double*)sum_out, partial);
+  hipLaunchKernelGGL(k_dev2<T>, dim3(g), dim3(kT), 0, st, a, n, (const double*)sum_out, partial);
   PMS_LAUNCH_CHECK();
   sum_partials(partial, g, 1, dev2_out, nullptr, st);
+}
+void mean_std(const float* a, size_t n, double* partial, double* sum_out, double* dev2_out, hipStream_t st) {
+  mean_std_t(a, n, partial, sum_out, dev2_out, st);
+}
+void mean_std(const double* a, size_t n, double* partial, double* sum_out, double* dev2_out, hipStream_t st) {
+  mean_std_t(a, n, partial, sum_out, dev2_out, st);
 }
 
 }  // namespace pms

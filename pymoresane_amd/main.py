@@ -48,11 +48,16 @@ def decode_trace(events, n):
     return out
 
 
+PRECISIONS = {"float32": 0, "float64": 1}
+
+
 def make_params(n, psf_n, subregion, scale_count, sigma_level, loop_gain, tolerance, accuracy, major_loop_miter,
                 minor_loop_miter, conv_mode, enforce_positivity, edge_suppression, edge_offset, flux_threshold,
-                neg_comp, edge_excl, int_excl):
+                neg_comp, edge_excl, int_excl, precision="float32"):
     if conv_mode not in ("linear", "circular"):
         raise ValueError("conv_mode must be 'linear' or 'circular', got %r" % (conv_mode,))
+    if precision not in PRECISIONS:
+        raise ValueError("precision must be 'float32' or 'float64', got %r" % (precision,))
     prm = MoresaneParams()
     prm.n, prm.psf_n = int(n), int(psf_n)
     prm.subregion = 0 if subregion is None else int(subregion)
@@ -67,6 +72,7 @@ def make_params(n, psf_n, subregion, scale_count, sigma_level, loop_gain, tolera
     prm.flux_threshold = float(flux_threshold)
     prm.neg_comp = int(bool(neg_comp))
     prm.edge_excl, prm.int_excl = int(edge_excl), int(int_excl)
+    prm.precision = PRECISIONS[precision]
     return prm
 
 
@@ -137,28 +143,35 @@ class DataImage(object):
     def moresane(self, subregion=None, scale_count=None, sigma_level=4, loop_gain=0.1, tolerance=0.75, accuracy=1e-6,
                  major_loop_miter=100, minor_loop_miter=30, all_on_gpu=False, decom_mode="hip", core_count=1,
                  conv_device="hip", conv_mode="linear", extraction_mode="hip", enforce_positivity=False,
-                 edge_suppression=False, edge_offset=0, flux_threshold=0, neg_comp=False, edge_excl=0, int_excl=0):
-        """Primary deconvolution (pymoresane/main.py:53-521); updates self.model and self.residual."""
+                 edge_suppression=False, edge_offset=0, flux_threshold=0, neg_comp=False, edge_excl=0, int_excl=0,
+                 precision="float32"):
+        """Primary deconvolution (pymoresane/main.py:53-521); updates self.model and self.residual.
+
+        ``precision`` (not in the reference): ``'float32'`` runs the wavelet planes, convolutions, CG
+        vectors, model and residual in float32 with float64 inner products, step sizes and norms;
+        ``'float64'`` runs all device arithmetic in float64, the arithmetic of the reference's serial
+        numpy path, at about twice the memory traffic.
+        """
         logger.info("Starting...")
         self._check_modes(all_on_gpu, decom_mode, conv_device, extraction_mode)
         self._check_shapes()
         prm = make_params(self.dirty_data_shape[0], self.psf_data_shape[0], subregion, scale_count, sigma_level,
                           loop_gain, tolerance, accuracy, major_loop_miter, minor_loop_miter, conv_mode,
                           enforce_positivity, edge_suppression, edge_offset, flux_threshold, neg_comp, edge_excl,
-                          int_excl)
+                          int_excl, precision)
         self._run(False, 0, 0, prm)
 
     def moresane_by_scale(self, start_scale=1, stop_scale=20, subregion=None, sigma_level=4, loop_gain=0.1,
                           tolerance=0.75, accuracy=1e-6, major_loop_miter=100, minor_loop_miter=30, all_on_gpu=False,
                           decom_mode="hip", core_count=1, conv_device="hip", conv_mode="linear",
                           extraction_mode="hip", enforce_positivity=False, edge_suppression=False, edge_offset=0,
-                          flux_threshold=0, neg_comp=False, edge_excl=0, int_excl=0):
-        """Scale-by-scale deconvolution (pymoresane/main.py:523-599)."""
+                          flux_threshold=0, neg_comp=False, edge_excl=0, int_excl=0, precision="float32"):
+        """Scale-by-scale deconvolution (pymoresane/main.py:523-599); ``precision`` as for moresane."""
         self._check_modes(all_on_gpu, decom_mode, conv_device, extraction_mode)
         self._check_shapes()
         prm = make_params(self.dirty_data_shape[0], self.psf_data_shape[0], subregion, None, sigma_level, loop_gain,
                           tolerance, accuracy, major_loop_miter, minor_loop_miter, conv_mode, enforce_positivity,
-                          edge_suppression, edge_offset, flux_threshold, neg_comp, edge_excl, int_excl)
+                          edge_suppression, edge_offset, flux_threshold, neg_comp, edge_excl, int_excl, precision)
         self._run(True, start_scale, stop_scale, prm)
         self.complete = False
 
@@ -280,7 +293,7 @@ def main(argv=None):
                   extraction_mode=args.extractionmode, enforce_positivity=args.enforcepositivity,
                   edge_suppression=args.edgesuppression, edge_offset=args.edgeoffset,
                   flux_threshold=args.fluxthreshold, neg_comp=args.negcomp, edge_excl=args.edgeexcl,
-                  int_excl=args.intexcl)
+                  int_excl=args.intexcl, precision=args.precision)
     if args.singlerun:
         data.moresane(scale_count=args.scalecount, **common)
     else:

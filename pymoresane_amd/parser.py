@@ -65,6 +65,10 @@ def make_parser():
                    help="Pixels to exclude from the edges when estimating the noise.")
     p.add_argument("-ie", "--intexcl", type=int, default=0,
                    help="Pixels to exclude from the centre when estimating the noise.")
+    # not in the reference: the arithmetic of the device path
+    p.add_argument("-pr", "--precision", default="float32", choices=["float32", "float64"],
+                   help="Bulk arithmetic of the HIP path: float32 (float64 inner products and step sizes) or "
+                        "float64 throughout.")
     return p
 
 

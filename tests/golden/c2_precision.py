@@ -1,7 +1,12 @@
-"""Float64 oracle run of the 2048^2 by-scale precision case, and its comparison with a GPU result.
+"""Float64 oracle run of the 2048^2 by-scale precision case, its committed fixture, and comparisons.
 
     python tests/golden/c2_precision.py ORACLE.npz            # run the oracle (~30 min, CPU)
     python tests/golden/c2_precision.py ORACLE.npz GPU.npz    # compare (oracle file must exist)
+    python tests/golden/c2_precision.py ORACLE.npz --fixture tests/golden/c2_oracle.npz
+
+The fixture keeps what a GPU test needs without the 32-minute oracle run: the decision trace, the
+model (sparse: flat indices and float64 values of its non-zero pixels), and, as a cross-check of
+the residual the test recomputes from that model, the residual's norm, sum and every 256th row.
 
 GPU.npz is written on an MI355X by ``python bench.py --precision GPU.npz`` (same generator,
 seed 2: 2048^2 dirty, 4096^2 PSF, 2000 points + 50 blobs, noise 1e-4; moresane_by_scale 1..6
@@ -68,14 +73,28 @@ def compare(oracle_path, gpu_path):
     return out
 
 
+def write_fixture(oracle_path, fixture_path):
+    o = np.load(oracle_path)
+    model, residual = o["model"], o["residual"]
+    idx = np.flatnonzero(model)
+    np.savez_compressed(fixture_path, shape=np.array(model.shape, dtype=np.int64), kw=np.array(json.dumps(KW)),
+                        model_idx=idx.astype(np.int32), model_val=model.ravel()[idx].astype(np.float64),
+                        trace=o["trace"], residual_norm=np.float64(np.linalg.norm(residual)),
+                        residual_sum=np.float64(residual.sum()), residual_rows=residual[::256].astype(np.float64))
+    print("fixture: %d model pixels, %d bytes" % (idx.size, os.path.getsize(fixture_path)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("oracle")
     ap.add_argument("gpu", nargs="?")
     ap.add_argument("--perturb", type=float, default=0.0)
+    ap.add_argument("--fixture", metavar="OUT_NPZ", help="write the committed GPU-test fixture from ORACLE")
     args = ap.parse_args()
     if not os.path.exists(args.oracle):
         run_oracle(args.oracle, args.perturb)
+    if args.fixture:
+        write_fixture(args.oracle, args.fixture)
     if args.gpu:
         compare(args.oracle, args.gpu)
 

@@ -60,7 +60,7 @@ def test_spectrum_layout_conversion_is_a_permutation():
     """The host-side rfft2 -> library layout conversion moves every bin exactly once (no GPU)."""
     import numpy as np
     from pymoresane_amd.iuwt_convolution import spectrum_positions
-    for p in (4, 8, 32, 512, 8192):
+    for p in (4, 8, 32, 512, 8192, 6, 96, 1000, 6000):
         pos = spectrum_positions(p)
         assert sorted(pos.tolist()) == list(range(p))
 
@@ -70,9 +70,13 @@ def test_bad_arguments_fail_loudly_without_gpu():
     out = (ctypes.c_float * 4)()
     st = lib.pms_spectrum_from_rfft2(None, 8, out)
     assert st != 0 and b"NULL" in lib.pms_last_error()
-    src = (ctypes.c_float * (2 * 6 * 4))()
-    st = lib.pms_spectrum_from_rfft2(src, 6, out)  # not a power of two
-    assert st == -1 and b"power of two" in lib.pms_last_error()
+    p = 34  # 2 x 17: a prime factor above 13
+    src = (ctypes.c_float * (2 * p * (p // 2 + 1)))()
+    big = (ctypes.c_float * (2 * p * (p // 2 + 1)))()
+    st = lib.pms_spectrum_from_rfft2(src, p, big)
+    assert st == -1 and b"not supported" in lib.pms_last_error()
+    st = lib.pms_spectrum_from_rfft2(src, 7, big)  # odd frame side
+    assert st == -1 and b"not supported" in lib.pms_last_error()
 
 
 def test_modes_are_validated_before_any_device_work():

@@ -34,8 +34,11 @@ def test_parser_defaults_follow_the_reference():
     assert b.singlerun and b.scalecount == 5 and b.enforcepositivity and b.negcomp and b.convmode == "linear"
     assert b.decommode == "gpu" and (b.edgeexcl, b.intexcl, b.subregion, b.fluxthreshold) == (3, 4, 128, 0.5)
     assert b.minorloopmiter == 30
+    assert a.precision == "float32" and handle_parser(["d.fits", "p.fits", "-pr", "float64"]).precision == "float64"
     with pytest.raises(SystemExit):
         handle_parser(["d.fits", "p.fits", "-cm", "spherical"])
+    with pytest.raises(SystemExit):
+        handle_parser(["d.fits", "p.fits", "-pr", "float16"])
 
 
 def test_main_requires_output_names():
@@ -82,6 +85,7 @@ def test_main_forwards_arguments_and_writes_outputs(tmp_path, monkeypatch):
     assert kw["start_scale"] == 1 and kw["stop_scale"] == 6 and kw["loop_gain"] == 0.3
     assert kw["enforce_positivity"] and kw["conv_mode"] == "circular" and kw["minor_loop_miter"] == 50
     assert kw["decom_mode"] == kw["conv_device"] == kw["extraction_mode"] == "hip"
+    assert kw["precision"] == "float32"
     for suffix, val in (("_model.fits", 1.0), ("_residual.fits", 2.0), ("_restored.fits", 3.0)):
         _, a = fits.read(out + suffix)
         assert a.shape == (1, 1, 8, 8) and np.all(a == val)

@@ -167,9 +167,23 @@ def test_iuwt_constant_image_and_device_inputs(hip):
     assert np.max(np.abs(sm - 2.5)) < 1e-6
 
 
-def test_iuwt_rejects_too_many_scales(hip):
-    with pytest.raises(Exception, match="2\\^\\(scale\\+1\\)"):
-        iuwt.iuwt_decomposition(np.zeros((16, 16), np.float32), 5)
+@pytest.mark.parametrize("shape,scales,adjust", [((16, 16), 6, 0), ((48, 80), 7, 1), ((512, 384), 9, 0),
+                                                  ((1024, 300), 10, 2)])
+def test_iuwt_scales_beyond_the_side_match_oracle(hip, shape, scales, adjust):
+    """Scales whose taps exceed a side (2^(s+1) > side): the reference's clipped slices read that axis
+    reversed (iuwt.py:166-187); decomposition and recomposition match the oracle, which is pinned to
+    the reference's slice arithmetic, and the round trip is exact."""
+    say("iuwt over-long scales %s S=%d adj=%d" % (shape, scales, adjust))
+    x = np.random.default_rng(2).standard_normal(shape).astype(np.float32)
+    det, sm = iuwt.iuwt_decomposition(x, scales, adjust, store_smoothed=True)
+    rdet, rsm = oiuwt.iuwt_decomposition(x, scales, adjust, store_smoothed=True)
+    for k in range(det.shape[0]):
+        assert rel(det[k], rdet[k]) < 1e-5, (k, rel(det[k], rdet[k]))
+    assert rel(sm, rsm) < 1e-6
+    rec = iuwt.iuwt_recomposition(det, adjust)
+    assert rel(rec, oiuwt.iuwt_recomposition(rdet, adjust)) < 1e-5
+    if adjust == 0:
+        assert rel(iuwt.iuwt_recomposition(det, 0, smoothed_array=sm), x) < 1e-5
 
 
 # ------------------------------------------------------------------------------------------- FFT
@@ -185,6 +199,47 @@ def test_fft_convolve_matches_oracle(hip, n, mode):
     ref = oconv.fft_convolve(x, oconv.psf_spectrum(psf), mode)
     assert got.dtype == np.float32 and got.shape == (n, n)
     assert rel(got, ref) < 2e-6, rel(got, ref)
+
+
+# Frames whose side is not a power of two (the reference's numpy rfft2 takes any size,
+# iuwt_convolution.py:64-73): mixed-radix transforms with every supported prime (2, 3, 5, 7, 11, 13).
+@pytest.mark.parametrize("n,mode", [(6, "linear"), (48, "linear"), (52, "circular"), (88, "linear"),
+                                    (96, "linear"), (96, "circular"), (98, "linear"), (160, "linear"),
+                                    (160, "circular"), (480, "linear"), (500, "linear"), (1000, "linear"),
+                                    (1000, "circular"), (3000, "linear"), (3000, "circular")])
+def test_fft_convolve_any_even_side_matches_oracle(hip, n, mode):
+    say("fft_convolve n=%d %s (mixed radix)" % (n, mode))
+    rng = np.random.default_rng(31)
+    x = rng.standard_normal((n, n)).astype(np.float32)
+    p = 2 * n if mode == "linear" else n
+    psf = rng.standard_normal((p, p)).astype(np.float32)
+    got = conv.fft_convolve(x, conv.psf_spectrum(psf), conv_mode=mode)
+    ref = oconv.fft_convolve(x, oconv.psf_spectrum(psf), mode)
+    assert got.dtype == np.float32 and got.shape == (n, n)
+    assert rel(got, ref) < 5e-6, rel(got, ref)
+
+
+@pytest.mark.parametrize("p", [12, 96, 200, 1000])
+def test_psf_spectrum_any_even_side_round_trip(hip, p):
+    """The device spectrum of a non-power-of-two frame, read back in numpy rfft2 layout, equals
+    numpy's rfft2 of the frame, and a numpy spectrum converted to the library layout convolves like
+    the device-built one."""
+    say("psf spectrum p=%d (mixed radix)" % p)
+    rng = np.random.default_rng(32)
+    psf = rng.standard_normal((p, p)).astype(np.float32)
+    s = conv.gpu_r2c_fft(psf)
+    assert rel(s, np.fft.rfft2(psf.astype(np.float64))) < 2e-6
+    n = p // 2
+    x = rng.standard_normal((n, n)).astype(np.float32)
+    a = conv.fft_convolve(x, conv.psf_spectrum(psf), conv_mode="linear")
+    b = conv.fft_convolve(x, np.fft.rfft2(psf), conv_mode="linear")
+    assert rel(a, b) < 2e-6
+
+
+def test_fft_convolve_rejects_large_prime_factors(hip):
+    x = np.zeros((34, 34), np.float32)  # P = 68 = 4 * 17
+    with pytest.raises(Exception, match="prime factor"):
+        conv.fft_convolve(x, conv.psf_spectrum(np.zeros((68, 68), np.float32)), conv_mode="linear")
 
 
 def test_fft_convolve_accepts_numpy_spectrum_and_padded_psf(hip):
@@ -262,7 +317,11 @@ def test_fft_convolve_circular_delta_response(hip, n):
 
 # ------------------------------------------------------------------------------------ threshold
 @pytest.mark.parametrize("shape,ee,ie", [((64, 64), 0, 0), ((63, 64), 0, 0), ((64, 64), 5, 0), ((64, 64), 0, 7),
-                                         ((64, 64), 3, 9), ((256, 256), 0, 0), ((1024, 1024), 0, 0)])
+                                         ((64, 64), 3, 9), ((256, 256), 0, 0), ((1024, 1024), 0, 0),
+                                         # non-square: the excluded square is centred on the row count
+                                         ((48, 80), 0, 7), ((80, 48), 3, 9),
+                                         # h*w % 4 != 0: planes after the first start unaligned
+                                         ((33, 47), 0, 0), ((33, 47), 2, 5)])
 def test_estimate_threshold_is_exact(hip, shape, ee, ie):
     say("estimate_threshold %s edge=%d int=%d" % (shape, ee, ie))
     rng = np.random.default_rng(5)
@@ -471,6 +530,45 @@ def test_moresane_with_mask(hip, sub):
     img.moresane(**kw)
     ref = omain.OracleImage(dirty, psf, mask)
     ref.moresane(**kw)
+    assert discrete(img.trace) == discrete(ref.trace)
+    assert rel(img.model, ref.model) < 1e-4
+    assert rel(img.residual, ref.residual) < 1e-4
+
+
+@pytest.mark.parametrize("n,mode", [(96, "linear"), (160, "circular"), (1000, "linear")])
+def test_moresane_any_even_side(hip, n, mode):
+    """Deconvolution of images whose side is not a power of two (FFT frames 192, 160 and 2000):
+    identical decision trace, model and residual within 1e-4 of the float64 oracle."""
+    say("moresane %d^2 %s (mixed-radix frames)" % (n, mode))
+    dirty, psf = synth.make_pair(n, 40 + n, psf_factor=2 if mode == "linear" else 1, n_points=12, n_blobs=2)
+    kw = dict(scale_count=4, loop_gain=0.2, major_loop_miter=8 if n < 1000 else 3, minor_loop_miter=20,
+              enforce_positivity=True, conv_mode=mode)
+    img, ref = run_pair(dirty, psf, **kw)
+    assert any(e[0] == "update" for e in ref.trace)
+    assert discrete(img.trace) == discrete(ref.trace)
+    assert rel(img.model, ref.model) < 1e-4
+    assert rel(img.residual, ref.residual) < 1e-4
+
+
+@pytest.mark.parametrize("by_scale", [False, True])
+def test_moresane_small_subregion_automatic_scales(hip, by_scale):
+    """A 64^2 subregion of a 256^2 image with scale_count=None: the scale count comes from the whole
+    image (log2(256) - 1 = 7, main.py:108-109), so the coarsest scales of the subregion have taps
+    longer than its side; by scale, the run goes on through those scale counts.  Identical trace,
+    model and residual within 1e-4 of the oracle."""
+    say("moresane 256^2, subregion 64, automatic scales (by_scale=%s)" % by_scale)
+    from pymoresane_amd.main import DataImage
+    dirty, psf = synth.make_pair(256, 61, n_points=20, n_blobs=2)
+    kw = dict(subregion=64, loop_gain=0.2, major_loop_miter=6, minor_loop_miter=20, enforce_positivity=True)
+    img = DataImage(dirty, psf, None, 1.0, 1.0)
+    ref = omain.OracleImage(dirty, psf)
+    if by_scale:
+        img.moresane_by_scale(start_scale=5, stop_scale=8, **kw)
+        ref.moresane_by_scale(start_scale=5, stop_scale=8, **kw)
+    else:
+        img.moresane(scale_count=None, **kw)
+        ref.moresane(scale_count=None, **kw)
+    assert len(ref.trace) > 0
     assert discrete(img.trace) == discrete(ref.trace)
     assert rel(img.model, ref.model) < 1e-4
     assert rel(img.residual, ref.residual) < 1e-4
