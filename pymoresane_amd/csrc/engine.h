@@ -74,6 +74,9 @@ struct JobSizes {
 // device bytes one job of this geometry needs (its workspace plus nothing else)
 size_t job_footprint_bytes(const Geometry& g);
 
+// Minor-loop iterations a runner queues per host step unless PMS_CG_ITERS says otherwise.
+constexpr int kDefaultCgIters = 4;
+
 // Per-job scalar slot (doubles) in the runner's scalar block: the only values read back.
 constexpr int kSlotDoubles = 64;
 constexpr int kSlotInts = 64;
@@ -312,7 +315,8 @@ class Runner {
   double wait_seconds_ = 0.0;
   bool use_graph_ = false;
   bool snr_direct_ = false;
-  std::map<std::vector<IterArgs>, hipGraphExec_t> graphs_;
+  int cg_iters_ = 1;
+  std::map<std::pair<int, std::vector<IterArgs>>, hipGraphExec_t> graphs_;
 };
 
 }  // namespace pms

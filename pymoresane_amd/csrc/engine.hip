@@ -34,11 +34,16 @@ constexpr int kT = 256;
 
 // Scalar slot layout (doubles): [0,16) thresholds, [16,32) normalised maxima, [32,48) PSF scale
 // energies, 48/51 rr and rnrn (swapped every iteration), 49 p.Ap, 50 alpha, 52 beta, 53 SNR
-// residual sum, 54 SNR energy sum, 56/57 mean sum and squared deviations, float at 60: dmax.
-// Int slot: [0] clip flag (written only for jobs that enforce positivity), [16,32) suppression borders.
+// residual sum, 54 SNR energy sum, 56/57 mean sum and squared deviations, 58/59 the minor loop's
+// current and previous SNR, 60: dmax (in the job's element type), 61/62 the minor loop's iteration
+// count and exit code (-1 while it iterates) as read back by the host.
+// Int slot: [0] clip flag (written only for jobs that enforce positivity), [1] minor loop still
+// iterating, [2] its iteration count, [3] its exit code, [16,32) suppression borders.
 constexpr int kThr = 0, kNsm = 16, kEnergy = 32, kRR = 48, kPAP = 49, kAlpha = 50, kRNRN = 51, kBeta = 52,
-              kD2 = 53, kE2 = 54, kMeanSum = 56, kDev2 = 57, kDmax = 60;
-constexpr int kClip = 0, kBorders = 16;
+              kD2 = 53, kE2 = 54, kMeanSum = 56, kDev2 = 57, kSnrCur = 58, kSnrLast = 59, kDmax = 60,
+              kNiterD = 61, kExitD = 62;
+// Int slot: [0] clip flag, [1] minor loop active, [2] minor iterations, [3] exit code, [16,32) borders.
+constexpr int kClip = 0, kActive = 1, kNiter = 2, kExit = 3, kBorders = 16;
 
 // 16 bytes of plane elements: four floats or two doubles (the side is even, so a plane holds a
 // multiple of four elements and every job's vector starts 16-byte aligned).
@@ -54,7 +59,7 @@ template <class T>
 struct CgJob {
   T *x, *xn, *r, *rn, *p, *ap;
   double *partial, *scal, *rr, *rnrn;
-  int* iflag;
+  int* iflag;  // iflag[kActive] == 0: the job's minor loop has exited, its kernels do nothing
   int enforce;
 };
 template <class T>
@@ -102,6 +107,7 @@ __device__ __forceinline__ T block_min_all(const double* __restrict__ partial, i
 template <class T>
 __global__ __launch_bounds__(kT) void k_cg_pap(CgBatch<T> b) {
   const CgJob<T>& J = b.j[blockIdx.y];
+  if (J.iflag[kActive] == 0) return;
   const V16<T>* __restrict__ p = reinterpret_cast<const V16<T>*>(J.p);
   const V16<T>* __restrict__ ap = reinterpret_cast<const V16<T>*>(J.ap);
   const size_t nv = b.n / kLanes<T>;
@@ -119,6 +125,7 @@ __global__ __launch_bounds__(kT) void k_cg_pap(CgBatch<T> b) {
 template <class T>
 __global__ __launch_bounds__(kT) void k_cg_alpha_fin(CgBatch<T> b, int np) {
   const CgJob<T>& J = b.j[blockIdx.y];
+  if (J.iflag[kActive] == 0) return;
   const double tot = block_total(J.partial, np);
   if (threadIdx.x == 0) {
     J.scal[kPAP] = tot;
@@ -135,6 +142,7 @@ __device__ __forceinline__ T xn_of(double a, T p, T x) {
 template <class T>
 __global__ __launch_bounds__(kT) void k_cg_xn(CgBatch<T> b) {
   const CgJob<T>& J = b.j[blockIdx.y];
+  if (J.iflag[kActive] == 0) return;
   const double a = J.scal[kAlpha];
   const V16<T>* __restrict__ x = reinterpret_cast<const V16<T>*>(J.x);
   const V16<T>* __restrict__ p = reinterpret_cast<const V16<T>*>(J.p);
@@ -161,6 +169,10 @@ __global__ __launch_bounds__(kT) void k_cg_xn(CgBatch<T> b) {
 template <class T>
 __global__ __launch_bounds__(kT) void k_cg_clip(CgBatch<T> b, int np) {
   const CgJob<T>& J = b.j[blockIdx.y];
+  if (J.iflag[kActive] == 0) {  // the clip pass's operator launches are predicated on this flag
+    if (blockIdx.x == 0 && threadIdx.x == 0) J.iflag[kClip] = 0;
+    return;
+  }
   const int flag = (J.enforce && block_min_all<T>(J.partial, np) < (T)0) ? 1 : 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) J.iflag[kClip] = flag;
   if (flag == 0) return;
@@ -192,6 +204,7 @@ __device__ __forceinline__ T rn_of(double a, T ap, T r) {
 template <class T>
 __global__ __launch_bounds__(kT) void k_cg_rn(CgBatch<T> b) {
   const CgJob<T>& J = b.j[blockIdx.y];
+  if (J.iflag[kActive] == 0) return;
   const double a = J.scal[kAlpha];
   const V16<T>* __restrict__ r = reinterpret_cast<const V16<T>*>(J.r);
   const V16<T>* __restrict__ ap = reinterpret_cast<const V16<T>*>(J.ap);
@@ -216,6 +229,7 @@ __global__ __launch_bounds__(kT) void k_cg_rn(CgBatch<T> b) {
 template <class T>
 __global__ __launch_bounds__(kT) void k_cg_p(CgBatch<T> b, int np) {
   const CgJob<T>& J = b.j[blockIdx.y];
+  if (J.iflag[kActive] == 0) return;
   const double tot = block_total_all(J.partial, np);
   const double be = tot / *J.rr;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -245,6 +259,7 @@ struct SnrJob {
   T* rs;
   const T* dcg;
   const double* scal;  // alpha at kAlpha
+  const int* active;
   double* partial;
   size_t n;  // elements: scales x plane (a multiple of 4)
   int nblk;  // blocks and partials of this job (a function of n only)
@@ -257,7 +272,7 @@ struct SnrBatch {
 template <class T>
 __global__ __launch_bounds__(kT) void k_cg_snr(SnrBatch<T> b) {
   const SnrJob<T>& J = b.j[blockIdx.y];
-  if ((int)blockIdx.x >= J.nblk) return;
+  if ((int)blockIdx.x >= J.nblk || *J.active == 0) return;
   const double a = J.scal[kAlpha];
   V16<T>* __restrict__ rs = reinterpret_cast<V16<T>*>(J.rs);
   const V16<T>* __restrict__ dcg = reinterpret_cast<const V16<T>*>(J.dcg);
@@ -274,6 +289,58 @@ __global__ __launch_bounds__(kT) void k_cg_snr(SnrBatch<T> b) {
   }
   acc = block_sum_f64(acc);
   if (threadIdx.x == 0) J.partial[blockIdx.x] = acc;
+}
+
+// Exit rules of the minor loop (main.py:427-457) on the device, one thread per job, after an
+// iteration's SNR sums: the SNR, the iteration count and, when a rule fires, the exit code; an exit
+// clears the job's active flag, which turns its later queued iterations into no-ops.  The host
+// reads the outcome after the step and replays the iterate swaps of the iterations that ran.
+struct CtlBatch {
+  double* scal[kMaxBatch];
+  int* iflag[kMaxBatch];
+  int miter[kMaxBatch];
+  int n;
+};
+
+__global__ void k_minor_ctl(CtlBatch b) {
+  const int j = threadIdx.x;
+  if (j >= b.n) return;
+  double* sc = b.scal[j];
+  int* fl = b.iflag[j];
+  if (fl[kActive] == 0) return;
+  const double snr = 20.0 * log10(sqrt(sc[kE2]) / sqrt(sc[kD2]));
+  const double last = sc[kSnrCur];
+  const int niter = fl[kNiter] + 1;
+  int code = -1;
+  if (niter == 1 && snr > 40)
+    code = EXIT_FIRST_HIGH;
+  else if (snr > 40)
+    code = EXIT_ACCEPT;
+  else if (niter > 2 && snr <= last)
+    code = snr > 10.5 ? EXIT_FELL_OK : EXIT_FELL_LOW;
+  else if (niter >= b.miter[j])
+    code = EXIT_CAP;
+  sc[kSnrLast] = last;
+  sc[kSnrCur] = snr;
+  sc[kNiterD] = (double)niter;
+  sc[kExitD] = (double)code;
+  fl[kNiter] = niter;
+  if (code >= 0) {
+    fl[kExit] = code;
+    fl[kActive] = 0;
+  }
+}
+
+// start of a minor loop: iterating, no iterations yet, SNRs 0 (main.py:374-377)
+__global__ void k_minor_reset(double* scal, int* iflag, int iterate) {
+  if (threadIdx.x != 0) return;
+  iflag[kActive] = iterate;
+  iflag[kNiter] = 0;
+  iflag[kExit] = EXIT_CAP;
+  scal[kSnrCur] = 0.0;
+  scal[kSnrLast] = 0.0;
+  scal[kNiterD] = 0.0;
+  scal[kExitD] = -1.0;
 }
 
 // model[r0+r, c0+c] += gain * x[r, c]
@@ -738,6 +805,8 @@ void Task::enqueue_post_t(hipStream_t st) {
         sum_partials(J.partial, reduce_grid(s2), 1, rr_, nullptr, st);
         plane_sumsq_partials(Es, 1, (size_t)ns * s2, J.partial, nullptr, st);
         sum_partials(J.partial, reduce_grid((size_t)ns * s2), 1, scal_ + kE2, nullptr, st);
+        hipLaunchKernelGGL(k_minor_reset, dim3(1), dim3(64), 0, st, scal_, iflag_, iterate_ ? 1 : 0);
+        PMS_LAUNCH_CHECK();
       }
       break;
     case UPDATE:
@@ -914,41 +983,34 @@ void Task::consume(const double* h) {
         minor_exit();
         break;
       }
-      const double d2 = h[kD2];
-      snr_last_ = snr_current_;
-      snr_current_ = 20.0 * std::log10(std::sqrt(e2_) / std::sqrt(d2));
-      ++niter_;
-      if (niter_ == 1 && snr_current_ > 40) {
-        ++min_scale_;
-        exit_code_ = EXIT_FIRST_HIGH;
-        minor_exit();
-        break;
+      // The step ran up to K iterations; the device applied the exit rules after each
+      // (k_minor_ctl).  Replay the iterate swaps of the iterations that ran: every iteration that
+      // did not exit swaps (x, xn), (r, rn), (rr, rnrn) (main.py:456-457), and so does the one that
+      // reaches the cap; an accepted exit takes xn as x (main.py:441).
+      const int niter = (int)h[kNiterD];
+      const int code = (int)h[kExitD];
+      const int ran = niter - niter_;
+      PMS_REQUIRE(ran >= 1, "minor loop step ran no iteration");
+      niter_ = niter;
+      snr_last_ = h[kSnrLast];
+      snr_current_ = h[kSnrCur];
+      const int swaps = (code < 0 || code == EXIT_CAP) ? ran : ran - 1;
+      for (int k = 0; k < swaps; ++k) {
+        std::swap(r_, rn_);
+        std::swap(x_, xn_);
+        std::swap(rr_, rnrn_);
       }
-      if (snr_current_ > 40) {
+      if (code < 0) break;  // still iterating: the next step queues more iterations
+      exit_code_ = code;
+      if (code == EXIT_FIRST_HIGH || code == EXIT_FELL_LOW) {
+        ++min_scale_;
+      } else if (code == EXIT_ACCEPT) {
         std::swap(x_, xn_);
         min_scale_ = 0;
-        exit_code_ = EXIT_ACCEPT;
-        minor_exit();
-        break;
+      } else if (code == EXIT_FELL_OK) {
+        min_scale_ = 0;
       }
-      if (niter_ > 2 && snr_current_ <= snr_last_) {
-        if (snr_current_ > 10.5) {
-          min_scale_ = 0;
-          exit_code_ = EXIT_FELL_OK;
-        } else {
-          ++min_scale_;
-          exit_code_ = EXIT_FELL_LOW;
-        }
-        minor_exit();
-        break;
-      }
-      std::swap(r_, rn_);
-      std::swap(x_, xn_);
-      std::swap(rr_, rnrn_);
-      if (niter_ >= spec.prm.minor_loop_miter) {
-        exit_code_ = EXIT_CAP;
-        minor_exit();
-      }
+      minor_exit();
       break;
     }
     case UPDATE: {
@@ -1112,6 +1174,9 @@ void Runner::refresh_options() {
   // main.py:413-424 writes them, instead of the incremental update (same quantity by linearity)
   const char* snr_env = getenv("PMS_SNR_DIRECT");
   snr_direct_ = snr_env && snr_env[0] == '1';
+  // PMS_CG_ITERS=K: minor-loop iterations queued per host step (the device applies the exit rules)
+  const char* k_env = getenv("PMS_CG_ITERS");
+  cg_iters_ = k_env ? std::max(1, std::min(32, atoi(k_env))) : kDefaultCgIters;
 }
 
 Runner::~Runner() {
@@ -1233,14 +1298,29 @@ void Runner::run_shared_t(SharedOpsT<T>& ops) {
   for (auto& kv : conv) conv_batch(kv.second.data(), (int)kv.second.size(), kv.first.first, kv.first.second, st_);
 }
 
+// The iterate buffers of iteration k of a step: every iteration that does not exit swaps (x, xn),
+// (r, rn) and (rr, rnrn), so odd iterations run on the swapped set.
+static std::vector<IterArgs> iteration_buffers(const std::vector<IterArgs>& its, int k) {
+  std::vector<IterArgs> out = its;
+  if (k & 1)
+    for (IterArgs& a : out) {
+      std::swap(a.x, a.xn);
+      std::swap(a.r, a.rn);
+      std::swap(a.rr, a.rnrn);
+    }
+  return out;
+}
+
 // Iterations of tasks that share an element type, plane shape and FFT frame run together, kMaxBatch
-// per launch.
+// per launch, cg_iters_ of them per step: the device's exit rules turn a finished job's remaining
+// iterations into no-ops, and the host reads the outcome once per step.
 void Runner::run_iterations(std::vector<Task*>& ts) {
   std::map<std::tuple<int, int, int>, std::vector<Task*>> parts;
   for (Task* t : ts) {
     const IterArgs a = t->iter_args();
     parts[std::make_tuple(a.f64, a.sub, a.p_sub)].push_back(t);
   }
+  const int K = cg_iters_;
   for (auto& kv : parts) {
     std::vector<Task*>& v = kv.second;
     for (size_t j0 = 0; j0 < v.size(); j0 += kMaxBatch) {
@@ -1249,15 +1329,17 @@ void Runner::run_iterations(std::vector<Task*>& ts) {
       std::vector<IterArgs> its;
       for (Task* t : chunk) its.push_back(t->iter_args());
       if (!use_graph_) {
-        enqueue_iterations(its, chunk);
+        for (int k = 0; k < K; ++k) enqueue_iterations(iteration_buffers(its, k), chunk);
         continue;
       }
-      // replay a captured graph of this exact set of buffers and scales (recorded on first use)
-      auto it = graphs_.find(its);
+      // replay a captured graph of these K iterations on this exact set of buffers and scales
+      // (recorded on first use)
+      const auto key = std::make_pair(K, its);
+      auto it = graphs_.find(key);
       if (it == graphs_.end()) {
         PMS_HIP(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
         try {
-          enqueue_iterations(its, chunk);
+          for (int k = 0; k < K; ++k) enqueue_iterations(iteration_buffers(its, k), chunk);
         } catch (...) {
           hipGraph_t junk = nullptr;
           (void)hipStreamEndCapture(st_, &junk);
@@ -1270,7 +1352,7 @@ void Runner::run_iterations(std::vector<Task*>& ts) {
         const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         PMS_HIP(e);
-        it = graphs_.emplace(its, exec).first;
+        it = graphs_.emplace(key, exec).first;
       }
       PMS_HIP(hipGraphLaunch(it->second, st_));
     }
@@ -1295,7 +1377,7 @@ void Runner::apply_op(const std::vector<Task*>& ts, const std::vector<IterArgs>&
     Job& J = *ts[i]->job();
     Planes<T>& B = J.pl<T>();
     const IterArgs& a = its[i];
-    const int* pred = clip_pass ? ts[i]->iflag() + kClip : nullptr;
+    const int* pred = ts[i]->iflag() + (clip_pass ? kClip : kActive);
     fj.push_back(typename EngineTypes<T>::Conv{(const T*)a.p, B.spec_sub, B.cbuf, B.fft_work, pred});
     dj.push_back(IuwtDecompJobT<T>{B.cbuf, a.max_scale, a.adj, B.Dcg, nullptr, B.iuwt_work, J.M + (size_t)a.adj * s2,
                                    nullptr, nullptr, pred, 0});
@@ -1356,13 +1438,15 @@ void Runner::enqueue_iterations_t(const std::vector<IterArgs>& its, const std::v
       Job& J = *ts[i]->job();
       Planes<T>& B = J.pl<T>();
       const IterArgs& a = its[i];
-      fj.push_back(typename EngineTypes<T>::Conv{(const T*)a.xn, B.spec_sub, B.cbuf, B.fft_work, nullptr});
+      const int* act = ts[i]->iflag() + kActive;
+      fj.push_back(typename EngineTypes<T>::Conv{(const T*)a.xn, B.spec_sub, B.cbuf, B.fft_work, act});
       dj.push_back(IuwtDecompJobT<T>{B.cbuf, a.max_scale, a.adj, nullptr, nullptr, B.iuwt_work,
-                                     J.M + (size_t)a.adj * s2, B.E + (size_t)a.adj * s2, J.partial, nullptr, 0});
+                                     J.M + (size_t)a.adj * s2, B.E + (size_t)a.adj * s2, J.partial, act, 0});
     }
     conv_batch(fj.data(), nb, sub, p_sub, st_);
     decompose_batch(dj.data(), nb, sub, sub, st_);
-    for (int i = 0; i < nb; ++i) sj.push_back(SumJob{ts[i]->job()->partial, dj[i].np, ts[i]->scal() + kD2});
+    for (int i = 0; i < nb; ++i)
+      sj.push_back(SumJob{ts[i]->job()->partial, dj[i].np, ts[i]->scal() + kD2, ts[i]->iflag() + kActive});
   } else {
     SnrBatch<T> sb{};
     int gmax = 1;
@@ -1372,14 +1456,24 @@ void Runner::enqueue_iterations_t(const std::vector<IterArgs>& its, const std::v
       const IterArgs& a = its[i];
       const size_t n = (size_t)(a.max_scale - a.adj) * s2;
       const int nblk = cg_grid(n);
-      sb.j[i] = SnrJob<T>{B.Rs, B.Dcg, ts[i]->scal(), J.partial, n, nblk};
+      sb.j[i] = SnrJob<T>{B.Rs, B.Dcg, ts[i]->scal(), ts[i]->iflag() + kActive, J.partial, n, nblk};
       gmax = std::max(gmax, nblk);
-      sj.push_back(SumJob{J.partial, nblk, ts[i]->scal() + kD2});
+      sj.push_back(SumJob{J.partial, nblk, ts[i]->scal() + kD2, ts[i]->iflag() + kActive});
     }
     hipLaunchKernelGGL(k_cg_snr<T>, dim3(gmax, nb), dim3(kT), 0, st_, sb);
     PMS_LAUNCH_CHECK();
   }
   sum_partials_batch(sj.data(), nb, st_);
+  // exit rules of every job of the chunk (main.py:427-457)
+  CtlBatch cb2{};
+  cb2.n = nb;
+  for (int i = 0; i < nb; ++i) {
+    cb2.scal[i] = ts[i]->scal();
+    cb2.iflag[i] = ts[i]->iflag();
+    cb2.miter[i] = ts[i]->spec.prm.minor_loop_miter;
+  }
+  hipLaunchKernelGGL(k_minor_ctl, dim3(1), dim3(64), 0, st_, cb2);
+  PMS_LAUNCH_CHECK();
 }
 
 }  // namespace pms

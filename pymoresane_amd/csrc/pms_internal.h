@@ -303,6 +303,7 @@ struct SumJob {
   const double* partial;
   long np;
   double* out;
+  const int* pred;  // skip the job when *pred == 0 (null = always)
 };
 void sum_partials_batch(const SumJob* jobs, int count, hipStream_t st);
 void plane_sumsq_partials(const float* a, int S, size_t plane, double* partial, const int* pred, hipStream_t st);

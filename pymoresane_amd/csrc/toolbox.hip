@@ -738,10 +738,12 @@ struct SumBatch {
   const double* partial[kMaxBatch];
   long np[kMaxBatch];
   double* out[kMaxBatch];
+  const int* pred[kMaxBatch];
 };
 
 // job blockIdx.x: same summation order as k_sum_partials for one group
 __global__ __launch_bounds__(kT) void k_sum_partials_batch(SumBatch b) {
+  if (pred_off(b.pred[blockIdx.x])) return;
   const double* p = b.partial[blockIdx.x];
   const long np = b.np[blockIdx.x];
   double acc = 0.0;
@@ -1103,6 +1105,7 @@ void sum_partials_batch(const SumJob* jobs, int count, hipStream_t st) {
       b.partial[k] = jobs[j0 + k].partial;
       b.np[k] = jobs[j0 + k].np;
       b.out[k] = jobs[j0 + k].out;
+      b.pred[k] = jobs[j0 + k].pred;
     }
     hipLaunchKernelGGL(k_sum_partials_batch, dim3(nb), dim3(kT), 0, st, b);
     PMS_LAUNCH_CHECK();

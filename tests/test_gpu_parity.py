@@ -630,3 +630,34 @@ def test_graph_replay_equals_eager(hip, monkeypatch):
     np.testing.assert_array_equal(graph.model, eager.model)
     np.testing.assert_array_equal(graph.residual, eager.residual)
     assert graph.trace == eager.trace
+
+
+@pytest.mark.parametrize("graph", ["0", "1"])
+def test_iterations_per_step_change_no_bit(hip, monkeypatch, graph):
+    """Minor-loop iterations queued K per host step, their exit rules applied on the device (a job's
+    iterations after its exit do nothing, the host replays the iterate swaps): K = 1, 2, 3, 8 give
+    the same models, residuals and traces bit for bit, eagerly and from captured graphs, plain and
+    by scale (whose minor loops end by every rule)."""
+    say("iterations per step, graph=%s" % graph)
+    from pymoresane_amd.main import DataImage
+    dirty, psf = synth.config_c0()
+    kw = dict(scale_count=4, loop_gain=0.2, sigma_level=4, major_loop_miter=30, tolerance=0.75,
+              enforce_positivity=True, conv_mode="linear", minor_loop_miter=30, accuracy=1e-6)
+    d2, p2 = synth.make_pair(128, 71, n_points=14, n_blobs=2)
+    monkeypatch.setenv("PMS_CG_GRAPH", graph)
+    out = []
+    for k in ("1", "2", "3", "8"):
+        monkeypatch.setenv("PMS_CG_ITERS", k)
+        a = DataImage(dirty, psf, None, 1.0, 1.0)
+        a.moresane(**kw)
+        b = DataImage(d2, p2, None, 1.0, 1.0)
+        b.moresane_by_scale(start_scale=1, stop_scale=5, loop_gain=0.2, major_loop_miter=10, minor_loop_miter=7,
+                            enforce_positivity=True)
+        out.append((a, b))
+    codes = {e[3] for _, b in out for e in b.trace if e[0] == "minor"}
+    assert len(codes) >= 3, codes  # several exit rules, the iteration cap among them
+    for a, b in out[1:]:
+        for x, y in ((a, out[0][0]), (b, out[0][1])):
+            np.testing.assert_array_equal(x.model, y.model)
+            np.testing.assert_array_equal(x.residual, y.residual)
+            assert x.trace == y.trace
