@@ -5,8 +5,9 @@
     python tests/golden/c2_precision.py ORACLE.npz --fixture tests/golden/c2_oracle.npz
 
 The fixture keeps what a GPU test needs without the 32-minute oracle run: the decision trace, the
-model (sparse: flat indices and float64 values of its non-zero pixels), and, as a cross-check of
-the residual the test recomputes from that model, the residual's norm, sum and every 256th row.
+model (sparse: gaps between the flat indices of its non-zero pixels, and their values rounded to
+float32), and, as a cross-check of the residual the test rebuilds from that model, the residual's
+norm, sum and every 256th row.
 
 GPU.npz is written on an MI355X by ``python bench.py --precision GPU.npz`` (same generator,
 seed 2: 2048^2 dirty, 4096^2 PSF, 2000 points + 50 blobs, noise 1e-4; moresane_by_scale 1..6
@@ -77,8 +78,11 @@ def write_fixture(oracle_path, fixture_path):
     o = np.load(oracle_path)
     model, residual = o["model"], o["residual"]
     idx = np.flatnonzero(model)
+    # flat indices as gaps (mostly 1: compresses to almost nothing), values rounded to float32 (6e-8
+    # relative, far below the 1e-4 bar); the test rebuilds the model with a cumulative sum
     np.savez_compressed(fixture_path, shape=np.array(model.shape, dtype=np.int64), kw=np.array(json.dumps(KW)),
-                        model_idx=idx.astype(np.int32), model_val=model.ravel()[idx].astype(np.float64),
+                        model_gaps=np.diff(idx, prepend=-1).astype(np.uint32),
+                        model_val=model.ravel()[idx].astype(np.float32),
                         trace=o["trace"], residual_norm=np.float64(np.linalg.norm(residual)),
                         residual_sum=np.float64(residual.sum()), residual_rows=residual[::256].astype(np.float64))
     print("fixture: %d model pixels, %d bytes" % (idx.size, os.path.getsize(fixture_path)))
