@@ -289,7 +289,7 @@ def bench_c3_class(side, max_batch=32, plain=False):
             "images_per_s": round(len(idx) / dt, 3)}
 
 
-def bench_served(count=60, max_batch=32, rate=20.0, budget=200e9):
+def bench_served(count=300, max_batch=32, rate=20.0, budget=200e9):
     """Poisson arrivals at `rate`/s, sides 512/1024/2048/4096 (50/30/15/5 %), S = log2(side) - 5."""
     import numpy as np
     from pymoresane_amd import batch, synth
@@ -334,26 +334,53 @@ def bench_served(count=60, max_batch=32, rate=20.0, budget=200e9):
     return out
 
 
-def run_precision(out_path, n=2048, stop_scale=6):
-    """2048^2 dirty / 4096^2 PSF (2000 points + 50 blobs, noise 1e-4), moresane_by_scale(1..6).
+C2_FIXTURE = os.path.join(ROOT, "tests", "golden", "c2_oracle.npz")
 
-    Writes model, residual and the decision trace to out_path (.npz) for comparison with the
-    float64 oracle run of the same case, and returns the timing line.
-    """
+
+def bench_c2(n=2048, stop_scale=6, out_path=None):
+    """c2, the precision form: 2048^2 dirty / 4096^2 PSF (2000 points + 50 blobs, noise 1e-4),
+    moresane_by_scale(1..6), in the float32 bulk mode and in the float64 mode, each against the float64
+    oracle's run of the same case (32 minutes on one core; its trace and model are the fixture
+    tests/golden/c2_oracle.npz, the residual rebuilt from that model as dirty - H model)."""
     import numpy as np
+    from scipy import signal
     from pymoresane_amd import synth
     from pymoresane_amd.main import DataImage
     dirty, psf = synth.config_c2(n=n)
-    img = DataImage(dirty, psf, None, 1, 1)
-    kw = dict(MORESANE_KW)
-    t0 = time.perf_counter()
-    img.moresane_by_scale(start_scale=1, stop_scale=stop_scale, **kw)
-    dt = time.perf_counter() - t0
-    np.savez(out_path, model=img.model, residual=img.residual,
-             trace=np.array(json.dumps([list(t) for t in img.trace])))
-    return {"workload": "c2 precision form: %dx%d dirty, %dx%d PSF, moresane_by_scale(1..%d)"
-                        % (n, n, 2 * n, 2 * n, stop_scale),
-            "seconds": round(dt, 3), "trace_events": len(img.trace), "output": out_path}
+    fx = np.load(C2_FIXTURE)
+    kw = json.loads(str(fx["kw"]))
+    model_o = np.zeros(tuple(fx["shape"]))
+    model_o.flat[np.cumsum(fx["model_gaps"].astype(np.int64)) - 1] = fx["model_val"]
+    resid_o = dirty.astype(np.float64) - signal.fftconvolve(model_o, psf.astype(np.float64))[n:2 * n, n:2 * n]
+    trace_o = json.loads(str(fx["trace"]))
+
+    def rel(a, b):
+        a = np.asarray(a, np.float64)
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    def decisions(trace):
+        return [tuple(v for v in e if not isinstance(v, float)) for e in trace]
+
+    rec = {"workload": "c2 precision form: %dx%d dirty, %dx%d PSF, moresane_by_scale(1..%d), against the "
+                       "float64 oracle run (%d decisions)" % (n, n, 2 * n, 2 * n, stop_scale, len(trace_o))}
+    for prec in ("float32", "float64"):
+        img = DataImage(dirty, psf, None, 1, 1)
+        t0 = time.perf_counter()
+        img.moresane_by_scale(start_scale=1, stop_scale=stop_scale, precision=prec, **kw)
+        dt = time.perf_counter() - t0
+        dg, do = decisions(img.trace), decisions([tuple(e) for e in trace_o])
+        first = next((i for i, (a, b) in enumerate(zip(dg, do)) if a != b), None if len(dg) == len(do) else
+                     min(len(dg), len(do)))
+        rec[prec] = {"seconds": round(dt, 3), "decisions": len(img.trace), "trace_equal": first is None,
+                     "first_divergence": first, "model_rel_l2": rel(img.model, model_o),
+                     "residual_rel_l2": rel(img.residual, resid_o)}
+        log("c2 %s: %.1f s, trace equal %s, model %.2e, residual %.2e" % (
+            prec, dt, first is None, rec[prec]["model_rel_l2"], rec[prec]["residual_rel_l2"]))
+        if out_path and prec == "float32":
+            np.savez(out_path, model=img.model, residual=img.residual,
+                     trace=np.array(json.dumps([list(t) for t in img.trace])))
+    rec["float64_cost"] = round(rec["float64"]["seconds"] / rec["float32"]["seconds"], 2)
+    return rec
 
 
 def main():
@@ -373,12 +400,14 @@ def main():
                     help=argparse.SUPPRESS)
     ap.add_argument("--c0-only", action="store_true", help="run only the c0 leg (with its CPU baseline unless "
                                                             "--no-cpu), print its line, exit")
-    ap.add_argument("--serve-requests", type=int, default=60)
+    ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--max-batch", type=int, default=32, help="jobs per lockstep batch (batched/served forms)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB float32 copy kernel 5 times (HBM counter calibration)")
-    ap.add_argument("--precision", metavar="OUT_NPZ",
-                    help="only run the 2048^2 by-scale precision case and write model/residual/trace to OUT_NPZ")
+    ap.add_argument("--precision", nargs="?", const="", metavar="OUT_NPZ",
+                    help="only run the 2048^2 by-scale precision case (float32 and float64 modes against the "
+                         "oracle fixture); with OUT_NPZ also write the float32 model/residual/trace there")
+    ap.add_argument("--no-c2", action="store_true", help="skip the c2 precision form")
     args = ap.parse_args()
     if args.cpu_mp_child:  # the mp baseline's child process: CPU only
         p_, s_, f_, w_ = args.cpu_mp_child
@@ -407,8 +436,8 @@ def main():
         ms = e0.elapsed_ms(e1) / 5
         print(json.dumps({"copy_bytes_each_way": 4 * count, "ms": ms, "GBps_read_plus_write": gbs(8 * count, ms)}))
         return
-    if args.precision:
-        print(json.dumps(run_precision(args.precision)))
+    if args.precision is not None:
+        print(json.dumps(bench_c2(out_path=args.precision or None)))
         return
     if args.c3_class:
         print(json.dumps(bench_c3_class(args.c3_class, args.max_batch, args.c3_plain)))
@@ -542,7 +571,10 @@ def main():
     check(lib.pms_stream_synchronize(st))
     for buf in (thr, tpl, ext, msk, tw_, xw_):
         buf.free()
-    dom_name = max(("iuwt_decompose", "iuwt_recompose", "fft_convolve"), key=lambda k: ops[k]["ms"])
+    # the roofline line reports the step operation furthest below the HBM roofline (lowest fraction);
+    # every step operation's fraction is listed beside it
+    step_ops = ("iuwt_decompose", "iuwt_recompose", "fft_convolve")
+    dom_name = min(step_ops, key=lambda k: ops[k]["frac_of_8TBps"])
     dom = ops[dom_name]
 
     # correctness of the measured step: perfect reconstruction and float64 spot check of the convolution
@@ -560,7 +592,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": dom["GBps"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(dom["GBps"] / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(dom_name, {"image": n, "scale_count": S, "psf": p}),
-                     "alg_bytes": dom["alg_bytes"], "launches": dom["launches"]},
+                     "alg_bytes": dom["alg_bytes"], "launches": dom["launches"],
+                     "step_ops_frac": {k: ops[k]["frac_of_8TBps"] for k in step_ops}},
         "ops": ops,
         "wall_ms_per_step": round(wall_ms, 4),
         "recon_rel_err": recon_err,
@@ -575,6 +608,9 @@ def main():
     if not args.no_c0:
         log("c0 plain deconvolution%s" % ("" if args.no_cpu else " and its float64 oracle run (1 core)"))
         res["c0"] = bench_c0(cpu=not args.no_cpu)
+    if not args.no_c2:
+        log("c2 precision form, float32 and float64 modes")
+        res["c2"] = bench_c2()
     if not args.no_cpu:
         log("cpu baseline (float64 oracle, 1 core, 2048^2 sample)")
         with one_core():
