@@ -103,6 +103,20 @@ size_t pms_threshold_work_bytes(int n_planes, int h, int w);
 int pms_estimate_threshold(const float* planes, int n_planes, int h, int w, int edge_excl, int int_excl,
                            double* thresholds_out, void* work, void* stream);
 
+/* pms_estimate_threshold without a workspace argument and without any device
+ * allocation: no counterpart in the reference; for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The median's select state and histograms live in one
+ * fixed device array per GPU (131,584 bytes, held until the process ends),
+ * so n_planes is limited to PMS_NOALLOC_MAX_PLANES; larger stacks are
+ * rejected with PMS_STATUS_ERR_ARG.  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (different streams or host threads)
+ * race and must be serialised by the caller.  Same results as
+ * pms_estimate_threshold, bit for bit. */
+#define PMS_NOALLOC_MAX_PLANES 16
+int pms_estimate_threshold_noalloc(const float* planes, int n_planes, int h, int w, int edge_excl, int int_excl,
+                                   double* thresholds_out, void* stream);
+
 /* out = planes where |planes| > sigma_level * thresholds[plane], else 0.
  * Replaces iuwt_toolbox.apply_threshold (pymoresane/iuwt_toolbox.py:50-75). */
 int pms_apply_threshold(const float* planes, int n_planes, int h, int w, const double* thresholds,

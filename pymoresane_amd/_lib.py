@@ -87,6 +87,7 @@ SIGNATURES = {
     "pms_fft_convolve": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pms_threshold_work_bytes": (c_size_t, [c_int, c_int, c_int]),
     "pms_estimate_threshold": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "pms_estimate_threshold_noalloc": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p]),
     "pms_apply_threshold": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_double, c_void_p, c_void_p, c_void_p]),
     "pms_extract_work_bytes": (c_size_t, [c_int, c_int, c_int]),
     "pms_source_extraction": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_int, c_void_p, c_void_p, c_void_p,

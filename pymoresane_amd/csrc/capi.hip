@@ -115,6 +115,7 @@ int pms_init(int device) {
     PMS_REQUIRE(device >= 0 && device < n, "device " + std::to_string(device) + " not present");
     PMS_HIP(hipSetDevice(device));
     fft_init_device(device);
+    (void)noalloc_mad_scratch();  // this device's address of the allocation-free median workspace
   });
 }
 
@@ -297,6 +298,26 @@ int pms_estimate_threshold(const float* planes, int n_planes, int h, int w, int 
     PMS_REQUIRE(planes && thresholds_out && work, "NULL buffer");
     PMS_REQUIRE(n_planes >= 1 && h >= 1 && w >= 1, "bad plane stack shape");
     NoiseMask nm = noise_mask(h, w, edge_excl, int_excl);
+    mad_work_init(work, n_planes, S(stream));
+    mad_thresholds(planes, n_planes, h, w, nm, thresholds_out, work, nullptr, S(stream));
+  });
+}
+
+int pms_estimate_threshold_noalloc(const float* planes, int n_planes, int h, int w, int edge_excl, int int_excl,
+                                   double* thresholds_out, void* stream) {
+  return guarded([&] {
+    PMS_REQUIRE(planes && thresholds_out, "NULL buffer");
+    PMS_REQUIRE(h >= 1 && w >= 1, "bad plane stack shape");
+    PMS_REQUIRE(n_planes >= 1 && n_planes <= PMS_NOALLOC_MAX_PLANES,
+                "pms_estimate_threshold_noalloc takes 1.." + std::to_string(PMS_NOALLOC_MAX_PLANES) +
+                    " planes per call (its fixed device workspace holds that many), got " +
+                    std::to_string(n_planes));
+    NoiseMask nm = noise_mask(h, w, edge_excl, int_excl);
+    PMS_REQUIRE(nm.count < (1L << 32), "pms_estimate_threshold_noalloc: more than 2^32 - 1 pixels per plane");
+    void* work = noalloc_mad_scratch();
+    PMS_REQUIRE(mad_work_bytes(n_planes) <= noalloc_mad_bytes(), "median workspace larger than the device array");
+    // the array keeps the previous call's contents: clear the histograms (the select state is
+    // rewritten by the first kernel of every call)
     mad_work_init(work, n_planes, S(stream));
     mad_thresholds(planes, n_planes, h, w, nm, thresholds_out, work, nullptr, S(stream));
   });

@@ -272,6 +272,9 @@ struct NoiseMask {
 };
 NoiseMask noise_mask(int h, int w, int edge_excl, int int_excl);
 size_t mad_work_bytes(int S);
+// the allocation-free median workspace of the current device (looked up once per device) and its size
+void* noalloc_mad_scratch();
+size_t noalloc_mad_bytes();
 void mad_work_init(void* work, int S, hipStream_t st);  // once per workspace (histograms start at 0)
 void mad_thresholds(const float* planes, int S, int h, int w, const NoiseMask& nm, double* thr, void* work,
                     const int* pred, hipStream_t st);

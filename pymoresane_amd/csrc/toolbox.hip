@@ -18,6 +18,8 @@
 // All reductions are deterministic: fixed grids, per-block partials in a
 // fixed order, integer atomics only.
 #include <algorithm>
+#include <map>
+#include <mutex>
 
 #include "pms_internal.h"
 
@@ -880,6 +882,33 @@ NoiseMask noise_mask(int h, int w, int edge_excl, int int_excl) {
 // workspace layout is independent of the plane count and element type of a call:
 // [select state: 16 slots of 32 bytes][histograms 16 x 2048]
 size_t mad_work_bytes(int) { return (size_t)kMaxPlanes * (kSelBytes + kBins * sizeof(uint32_t)); }
+
+// Allocation-free median workspace: one file-scope device array per GPU, holding the select state
+// and histograms of one call of up to kMaxPlanes planes (131,584 bytes).  Host code never names
+// the array in a kernel argument or copy: the array's host-side symbol is a stand-in, not a
+// device address, so each device's address is looked up once with hipGetSymbolAddress (with that
+// device current) and kept here.
+namespace {
+constexpr size_t kNoAllocMadBytes = (size_t)kMaxPlanes * (kSelBytes + kBins * sizeof(uint32_t));
+static __device__ __align__(256) unsigned char scratch[kNoAllocMadBytes];
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_ptr;
+}  // namespace
+
+size_t noalloc_mad_bytes() { return kNoAllocMadBytes; }
+
+void* noalloc_mad_scratch() {
+  int dev = 0;
+  PMS_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it != g_scratch_ptr.end()) return it->second;
+  void* ptr = nullptr;
+  PMS_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  PMS_REQUIRE(ptr != nullptr, "device scratch symbol has no address");
+  g_scratch_ptr.emplace(dev, ptr);
+  return ptr;
+}
 
 void mad_work_init(void* work, int, hipStream_t st) {
   uint32_t* hist = (uint32_t*)((char*)work + (size_t)kMaxPlanes * kSelBytes);
