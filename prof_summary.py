@@ -2,13 +2,13 @@
 
 Kernel trace (``--kernel-trace --stats``)::
 
-    python -m pymoresane_amd.prof_summary trace DIR/NAME_kernel_trace.csv DIR/NAME_kernel_stats.csv \
+    python prof_summary.py trace DIR/NAME_kernel_trace.csv DIR/NAME_kernel_stats.csv \
         --title TITLE --command "rocprofv3 ..." [--notes FILE] [--group NAME=REGEX ...] > SUMMARY.md
 
 HBM traffic (one ``--pmc FETCH_SIZE`` and one ``--pmc WRITE_SIZE`` run of the workload, and the same two
 of the plain copy kernel of known size for the counter calibration)::
 
-    python -m pymoresane_amd.prof_summary traffic FETCH.csv WRITE.csv COPY_FETCH.csv COPY_WRITE.csv \
+    python prof_summary.py traffic FETCH.csv WRITE.csv COPY_FETCH.csv COPY_WRITE.csv \
         --copy-bytes BYTES --op NAME=REGEX=LAUNCHES_PER_OP ... [--bench BENCH.json] \
         [--workload-key JSON] [--json OUT.json] --commands FILE > SUMMARY.md
 
