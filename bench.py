@@ -12,7 +12,8 @@ value = images/s (steps/s).  Per-op HIP-event timings give algorithmic HBM GB/s:
   8 N^2 + 3 * 8 P (P/2+1) bytes.
 The CPU baseline is the float64 oracle (single core) on a bounded sample: the same
 step on a 2048 x 2048 crop (S=7, linear convolution with a 4096 PSF), reported as
-4096^2-equivalent images/s (sample images/s divided by the pixel ratio 4).
+4096^2-equivalent images/s (the IUWT time scaled by the pixel ratio 4, the convolution
+time by the P^2 log P ratio).
 
 Also in the line: the batched (c3) and served (c4) forms, and c0 -- one plain 256^2 deconvolution beside
 the float64 oracle run of it on one core.
@@ -434,18 +435,23 @@ def main():
 
     lib = _lib.lib()
     if args.calibrate_runs64:
-        # a torch strided copy: each row of 32 floats contributes its first 16 (one 64-byte run)
-        import torch
-        rows = (128 << 20) // 32
-        a = torch.ones((rows, 32), dtype=torch.float32, device="cuda")
-        b = torch.empty((rows, 16), dtype=torch.float32, device="cuda")
-        torch.cuda.synchronize()
+        # the HIP runtime's pitched copy: the first 64 bytes of every 128-byte row of a 512 MiB buffer
+        # (one 64-byte run per row) into a dense 256 MiB buffer, 5 times
+        hip = ctypes.CDLL("libamdhip64.so")
+        hip.hipMemcpy2D.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
+                                    ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int]
+        hip.hipMemcpy2D.restype = ctypes.c_int
+        rows = (512 << 20) // 128
+        a, b2 = DeviceArray((rows * 32,)), DeviceArray((rows * 16,))
+        a.fill(1.0)
+        check(lib.pms_device_synchronize())
         t0 = time.perf_counter()
         for _ in range(5):
-            b.copy_(a[:, :16])
-        torch.cuda.synchronize()
+            if hip.hipMemcpy2D(b2.ptr, 64, a.ptr, 128, 64, rows, 3) != 0:  # 3: device to device
+                raise RuntimeError("hipMemcpy2D failed")
+        check(lib.pms_device_synchronize())
         ms = (time.perf_counter() - t0) * 1e3 / 5
-        print(json.dumps({"read_bytes_in_64B_runs": rows * 64, "write_bytes": rows * 64, "ms": ms}))
+        print(json.dumps({"read_bytes_in_64B_runs": rows * 64, "write_bytes": rows * 64, "ms": round(ms, 4)}))
         return
     if args.calibrate_copy:
         count = 64 << 20

@@ -74,7 +74,9 @@ struct JobSizes {
 // device bytes one job of this geometry needs (its workspace plus nothing else)
 size_t job_footprint_bytes(const Geometry& g);
 
-// Minor-loop iterations a runner queues per host step unless PMS_CG_ITERS says otherwise.
+// Minor-loop iterations a one-slot runner queues per host step unless PMS_CG_ITERS says otherwise.
+// A runner with more slots queues one: its batched launches and the other streams' work cover the
+// host round trip, while iterations queued past a job's exit still cost their (predicated) launches.
 constexpr int kDefaultCgIters = 4;
 
 // Per-job scalar slot (doubles) in the runner's scalar block: the only values read back.

@@ -44,6 +44,10 @@ constexpr int kThr = 0, kNsm = 16, kEnergy = 32, kRR = 48, kPAP = 49, kAlpha = 5
               kNiterD = 61, kExitD = 62;
 // Int slot: [0] clip flag, [1] minor loop active, [2] minor iterations, [3] exit code, [16,32) borders.
 constexpr int kClip = 0, kActive = 1, kNiter = 2, kExit = 3, kBorders = 16;
+// Upper region of a job's partial buffer (which holds at least 4096 doubles, and a CG reduction uses
+// at most 1024 blocks): the xn minima and the SNR partials, written by kernels that are still
+// reading the lower region's partials.
+constexpr int kPartialHi = 1024;
 
 // 16 bytes of plane elements: four floats or two doubles (the side is even, so a plane holds a
 // multiple of four elements and every job's vector starts 16-byte aligned).
@@ -121,29 +125,23 @@ __global__ __launch_bounds__(kT) void k_cg_pap(CgBatch<T> b) {
   if (threadIdx.x == 0) J.partial[blockIdx.x] = acc;
 }
 
-// alpha = rr / (p . Ap)
-template <class T>
-__global__ __launch_bounds__(kT) void k_cg_alpha_fin(CgBatch<T> b, int np) {
-  const CgJob<T>& J = b.j[blockIdx.y];
-  if (J.iflag[kActive] == 0) return;
-  const double tot = block_total(J.partial, np);
-  if (threadIdx.x == 0) {
-    J.scal[kPAP] = tot;
-    J.scal[kAlpha] = *J.rr / tot;
-  }
-}
-
 template <class T>
 __device__ __forceinline__ T xn_of(double a, T p, T x) {
   return (T)fma(a, (double)p, (double)x);
 }
 
-// xn = x + alpha p ; per-block minima of xn
+// alpha = rr / (p . Ap) from k_cg_pap's np partials (every block adds them in the same order; block 0
+// stores p . Ap and alpha) ; xn = x + alpha p ; per-block minima of xn in the upper partials
 template <class T>
-__global__ __launch_bounds__(kT) void k_cg_xn(CgBatch<T> b) {
+__global__ __launch_bounds__(kT) void k_cg_xn(CgBatch<T> b, int np) {
   const CgJob<T>& J = b.j[blockIdx.y];
   if (J.iflag[kActive] == 0) return;
-  const double a = J.scal[kAlpha];
+  const double tot = block_total_all(J.partial, np);
+  const double a = *J.rr / tot;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    J.scal[kPAP] = tot;
+    J.scal[kAlpha] = a;
+  }
   const V16<T>* __restrict__ x = reinterpret_cast<const V16<T>*>(J.x);
   const V16<T>* __restrict__ p = reinterpret_cast<const V16<T>*>(J.p);
   V16<T>* __restrict__ xn = reinterpret_cast<V16<T>*>(J.xn);
@@ -160,10 +158,10 @@ __global__ __launch_bounds__(kT) void k_cg_xn(CgBatch<T> b) {
     xn[i] = o;
   }
   m = block_min_t<T>(m);
-  if (threadIdx.x == 0) J.partial[blockIdx.x] = (double)m;
+  if (threadIdx.x == 0) J.partial[kPartialHi + blockIdx.x] = (double)m;
 }
 
-// positivity clip when enforce && min(xn) < 0 (the minimum from k_cg_xn's np per-block minima;
+// positivity clip when enforce && min(xn) < 0 (the minimum of k_cg_xn's np per-block minima;
 // block 0 stores the flag, which predicates the clip pass's operator launches):
 // xn[xn<0] = 0 ; p = (xn - x) / alpha
 template <class T>
@@ -173,7 +171,7 @@ __global__ __launch_bounds__(kT) void k_cg_clip(CgBatch<T> b, int np) {
     if (blockIdx.x == 0 && threadIdx.x == 0) J.iflag[kClip] = 0;
     return;
   }
-  const int flag = (J.enforce && block_min_all<T>(J.partial, np) < (T)0) ? 1 : 0;
+  const int flag = (J.enforce && block_min_all<T>(J.partial + kPartialHi, np) < (T)0) ? 1 : 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) J.iflag[kClip] = flag;
   if (flag == 0) return;
   const double a = J.scal[kAlpha];
@@ -200,28 +198,75 @@ __device__ __forceinline__ T rn_of(double a, T ap, T r) {
   return (T)fma(-a, (double)ap, (double)r);
 }
 
-// rn = r - alpha Ap ; per-block partials of rn . rn
+// Incremental SNR sums.  The SNR of main.py:413-424 compares E with M.W(H xn).  The convolution and
+// the decomposition are linear and xn = x + alpha p (the positivity clip rewrites p so that this
+// holds), so M.W(H xn) = M.W(H x) + alpha M.W(H p), and M.W(H p) is this iteration's apply_op
+// decomposition (Dcg).  The difference planes R = E - M.W(H x) are carried across the iterations of
+// a minor loop (R = E at its start, x = 0): R <- R - alpha Dcg, D2 = sum R^2.  This replaces one
+// convolution and one decomposition per iteration by one streaming pass over R and Dcg, done in the
+// same launch as the residual update.
 template <class T>
-__global__ __launch_bounds__(kT) void k_cg_rn(CgBatch<T> b) {
-  const CgJob<T>& J = b.j[blockIdx.y];
+struct RnJob {
+  const T* r;
+  const T* ap;
+  T* rn;
+  T* rs;         // SNR difference planes (null: the SNR sums come from elsewhere)
+  const T* dcg;  // this iteration's masked decomposition of H p
+  double* partial;
+  const double* scal;  // alpha at kAlpha
+  const int* iflag;
+  size_t n_snr;  // SNR elements: scales x plane (a multiple of 4)
+  int nblk_snr;  // blocks and partials of the SNR pass (a function of n_snr only)
+};
+template <class T>
+struct RnBatch {
+  RnJob<T> j[kMaxBatch];
+  size_t n;  // CG vector length
+  int np;    // blocks (and partials) of the rn pass: cg_grid(n)
+};
+
+// rn = r - alpha Ap with per-block partials of rn . rn in blocks [0, np) (lower partials), and
+// R <- R - alpha Dcg with per-block partials of R . R in blocks [0, nblk_snr) (upper partials).
+template <class T>
+__global__ __launch_bounds__(kT) void k_cg_rn(RnBatch<T> b) {
+  const RnJob<T>& J = b.j[blockIdx.y];
   if (J.iflag[kActive] == 0) return;
   const double a = J.scal[kAlpha];
-  const V16<T>* __restrict__ r = reinterpret_cast<const V16<T>*>(J.r);
-  const V16<T>* __restrict__ ap = reinterpret_cast<const V16<T>*>(J.ap);
-  V16<T>* __restrict__ rn = reinterpret_cast<V16<T>*>(J.rn);
-  const size_t nv = b.n / kLanes<T>;
-  double acc = 0.0;
-  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < nv; i += (size_t)gridDim.x * kT) {
-    const V16<T> u = ap[i], v = r[i];
-    V16<T> o;
+  if ((int)blockIdx.x < b.np) {
+    const V16<T>* __restrict__ r = reinterpret_cast<const V16<T>*>(J.r);
+    const V16<T>* __restrict__ ap = reinterpret_cast<const V16<T>*>(J.ap);
+    V16<T>* __restrict__ rn = reinterpret_cast<V16<T>*>(J.rn);
+    const size_t nv = b.n / kLanes<T>;
+    double acc = 0.0;
+    for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < nv; i += (size_t)b.np * kT) {
+      const V16<T> u = ap[i], v = r[i];
+      V16<T> o;
 #pragma unroll
-    for (int k = 0; k < kLanes<T>; ++k) o.a[k] = rn_of<T>(a, u.a[k], v.a[k]);
-    rn[i] = o;
+      for (int k = 0; k < kLanes<T>; ++k) o.a[k] = rn_of<T>(a, u.a[k], v.a[k]);
+      rn[i] = o;
 #pragma unroll
-    for (int k = 0; k < kLanes<T>; ++k) acc = fma((double)o.a[k], (double)o.a[k], acc);
+      for (int k = 0; k < kLanes<T>; ++k) acc = fma((double)o.a[k], (double)o.a[k], acc);
+    }
+    acc = block_sum_f64(acc);
+    if (threadIdx.x == 0) J.partial[blockIdx.x] = acc;
   }
-  acc = block_sum_f64(acc);
-  if (threadIdx.x == 0) J.partial[blockIdx.x] = acc;
+  if (J.rs && (int)blockIdx.x < J.nblk_snr) {
+    V16<T>* __restrict__ rs = reinterpret_cast<V16<T>*>(J.rs);
+    const V16<T>* __restrict__ dcg = reinterpret_cast<const V16<T>*>(J.dcg);
+    const size_t nv = J.n_snr / kLanes<T>;
+    double acc = 0.0;
+    for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < nv; i += (size_t)J.nblk_snr * kT) {
+      const V16<T> d = dcg[i];
+      V16<T> v = rs[i];
+#pragma unroll
+      for (int k = 0; k < kLanes<T>; ++k) v.a[k] = rn_of<T>(a, d.a[k], v.a[k]);
+      rs[i] = v;
+#pragma unroll
+      for (int k = 0; k < kLanes<T>; ++k) acc = fma((double)v.a[k], (double)v.a[k], acc);
+    }
+    acc = block_sum_f64(acc);
+    if (threadIdx.x == 0) J.partial[kPartialHi + blockIdx.x] = acc;
+  }
 }
 
 // rnrn = rn . rn (from k_cg_rn's np partials) ; beta = rnrn / rr (block 0 stores both) ;
@@ -248,66 +293,27 @@ __global__ __launch_bounds__(kT) void k_cg_p(CgBatch<T> b, int np) {
   }
 }
 
-// Incremental SNR sums.  The SNR of main.py:413-424 compares E with M.W(H xn).  The convolution and
-// the decomposition are linear and xn = x + alpha p (the positivity clip rewrites p so that this
-// holds), so M.W(H xn) = M.W(H x) + alpha M.W(H p), and M.W(H p) is this iteration's apply_op
-// decomposition (Dcg).  The difference planes R = E - M.W(H x) are carried across the iterations of
-// a minor loop (R = E at its start, x = 0): R <- R - alpha Dcg, D2 = sum R^2.  This replaces one
-// convolution and one decomposition per iteration by one streaming pass over R and Dcg.
-template <class T>
-struct SnrJob {
-  T* rs;
-  const T* dcg;
-  const double* scal;  // alpha at kAlpha
-  const int* active;
-  double* partial;
-  size_t n;  // elements: scales x plane (a multiple of 4)
-  int nblk;  // blocks and partials of this job (a function of n only)
-};
-template <class T>
-struct SnrBatch {
-  SnrJob<T> j[kMaxBatch];
-};
-
-template <class T>
-__global__ __launch_bounds__(kT) void k_cg_snr(SnrBatch<T> b) {
-  const SnrJob<T>& J = b.j[blockIdx.y];
-  if ((int)blockIdx.x >= J.nblk || *J.active == 0) return;
-  const double a = J.scal[kAlpha];
-  V16<T>* __restrict__ rs = reinterpret_cast<V16<T>*>(J.rs);
-  const V16<T>* __restrict__ dcg = reinterpret_cast<const V16<T>*>(J.dcg);
-  const size_t nv = J.n / kLanes<T>;
-  double acc = 0.0;
-  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < nv; i += (size_t)J.nblk * kT) {
-    const V16<T> d = dcg[i];
-    V16<T> r = rs[i];
-#pragma unroll
-    for (int k = 0; k < kLanes<T>; ++k) r.a[k] = rn_of<T>(a, d.a[k], r.a[k]);
-    rs[i] = r;
-#pragma unroll
-    for (int k = 0; k < kLanes<T>; ++k) acc = fma((double)r.a[k], (double)r.a[k], acc);
-  }
-  acc = block_sum_f64(acc);
-  if (threadIdx.x == 0) J.partial[blockIdx.x] = acc;
-}
-
-// Exit rules of the minor loop (main.py:427-457) on the device, one thread per job, after an
-// iteration's SNR sums: the SNR, the iteration count and, when a rule fires, the exit code; an exit
-// clears the job's active flag, which turns its later queued iterations into no-ops.  The host
-// reads the outcome after the step and replays the iterate swaps of the iterations that ran.
+// Exit rules of the minor loop (main.py:427-457) on the device, one block per job, after an
+// iteration's SNR partial sums: D2 (their total), the SNR, the iteration count and, when a rule
+// fires, the exit code; an exit clears the job's active flag, which turns its later queued
+// iterations into no-ops.  The host reads the outcome after the step and replays the iterate swaps
+// of the iterations that ran.
 struct CtlBatch {
   double* scal[kMaxBatch];
   int* iflag[kMaxBatch];
+  const double* partial[kMaxBatch];  // the SNR partials
+  long np[kMaxBatch];
   int miter[kMaxBatch];
-  int n;
 };
 
-__global__ void k_minor_ctl(CtlBatch b) {
-  const int j = threadIdx.x;
-  if (j >= b.n) return;
+__global__ __launch_bounds__(kT) void k_minor_ctl(CtlBatch b) {
+  const int j = blockIdx.x;
   double* sc = b.scal[j];
   int* fl = b.iflag[j];
   if (fl[kActive] == 0) return;
+  const double d2 = block_total(b.partial[j], (int)b.np[j]);
+  if (threadIdx.x != 0) return;
+  sc[kD2] = d2;
   const double snr = 20.0 * log10(sqrt(sc[kE2]) / sqrt(sc[kD2]));
   const double last = sc[kSnrCur];
   const int niter = fl[kNiter] + 1;
@@ -397,12 +403,6 @@ int cg_grid(size_t n) {
 template <class T, class K>
 void launch_cg(K kernel, const CgBatch<T>& b, int nb, hipStream_t st) {
   hipLaunchKernelGGL(kernel, dim3(cg_grid(b.n), nb), dim3(kT), 0, st, b);
-  PMS_LAUNCH_CHECK();
-}
-
-template <class T, class K>
-void launch_cg_fin(K kernel, const CgBatch<T>& b, int nb, hipStream_t st) {
-  hipLaunchKernelGGL(kernel, dim3(1, nb), dim3(kT), 0, st, b, cg_grid(b.n));
   PMS_LAUNCH_CHECK();
 }
 
@@ -1176,7 +1176,7 @@ void Runner::refresh_options() {
   snr_direct_ = snr_env && snr_env[0] == '1';
   // PMS_CG_ITERS=K: minor-loop iterations queued per host step (the device applies the exit rules)
   const char* k_env = getenv("PMS_CG_ITERS");
-  cg_iters_ = k_env ? std::max(1, std::min(32, atoi(k_env))) : kDefaultCgIters;
+  cg_iters_ = k_env ? std::max(1, std::min(32, atoi(k_env))) : (jobs_.size() == 1 ? kDefaultCgIters : 1);
 }
 
 Runner::~Runner() {
@@ -1395,7 +1395,8 @@ void Runner::enqueue_iterations(const std::vector<IterArgs>& its, const std::vec
     enqueue_iterations_t<float>(its, ts);
 }
 
-// One CG iteration (main.py:380-424) up to the SNR sums for up to kMaxBatch tasks of one element type.
+// One CG iteration (main.py:380-424), its SNR sums and its exit rules (main.py:427-457) for up to
+// kMaxBatch tasks of one element type.
 template <class T>
 void Runner::enqueue_iterations_t(const std::vector<IterArgs>& its, const std::vector<Task*>& ts) {
   const int nb = (int)ts.size();
@@ -1421,17 +1422,37 @@ void Runner::enqueue_iterations_t(const std::vector<IterArgs>& its, const std::v
   }
   apply_op<T>(ts, its, false);
   launch_cg(k_cg_pap<T>, cb, nb, st_);
-  launch_cg_fin(k_cg_alpha_fin<T>, cb, nb, st_);
-  launch_cg(k_cg_xn<T>, cb, nb, st_);
+  launch_cg_after(k_cg_xn<T>, cb, nb, st_);
   if (!clip_ts.empty()) {
     launch_cg_after(k_cg_clip<T>, clip_b, (int)clip_ts.size(), st_);
     apply_op<T>(clip_ts, clip_its, true);
   }
-  launch_cg(k_cg_rn<T>, cb, nb, st_);
+  // rn, and in the incremental form the SNR sums (main.py:413-424) in the same launch
+  RnBatch<T> rb{};
+  rb.n = s2;
+  rb.np = cg_grid(s2);
+  int gx = rb.np;
+  CtlBatch ctl{};
+  for (int i = 0; i < nb; ++i) {
+    Job& J = *ts[i]->job();
+    Planes<T>& B = J.pl<T>();
+    const IterArgs& a = its[i];
+    const size_t n = (size_t)(a.max_scale - a.adj) * s2;
+    const int nblk = cg_grid(n);
+    rb.j[i] = RnJob<T>{(const T*)a.r, (const T*)a.ap, (T*)a.rn, snr_direct_ ? nullptr : B.Rs, B.Dcg, J.partial,
+                       ts[i]->scal(), ts[i]->iflag(), n, nblk};
+    if (!snr_direct_) gx = std::max(gx, nblk);
+    ctl.scal[i] = ts[i]->scal();
+    ctl.iflag[i] = ts[i]->iflag();
+    ctl.partial[i] = J.partial + kPartialHi;
+    ctl.np[i] = nblk;
+    ctl.miter[i] = ts[i]->spec.prm.minor_loop_miter;
+  }
+  hipLaunchKernelGGL(k_cg_rn<T>, dim3(gx, nb), dim3(kT), 0, st_, rb);
+  PMS_LAUNCH_CHECK();
   launch_cg_after(k_cg_p<T>, cb, nb, st_);
-  // model sources and the SNR sums (main.py:413-424)
-  std::vector<SumJob> sj;
   if (snr_direct_) {
+    // main.py:413-424 as written: convolve xn, decompose it against the extracted sources
     std::vector<typename EngineTypes<T>::Conv> fj;
     std::vector<IuwtDecompJobT<T>> dj;
     for (int i = 0; i < nb; ++i) {
@@ -1445,34 +1466,13 @@ void Runner::enqueue_iterations_t(const std::vector<IterArgs>& its, const std::v
     }
     conv_batch(fj.data(), nb, sub, p_sub, st_);
     decompose_batch(dj.data(), nb, sub, sub, st_);
-    for (int i = 0; i < nb; ++i)
-      sj.push_back(SumJob{ts[i]->job()->partial, dj[i].np, ts[i]->scal() + kD2, ts[i]->iflag() + kActive});
-  } else {
-    SnrBatch<T> sb{};
-    int gmax = 1;
     for (int i = 0; i < nb; ++i) {
-      Job& J = *ts[i]->job();
-      Planes<T>& B = J.pl<T>();
-      const IterArgs& a = its[i];
-      const size_t n = (size_t)(a.max_scale - a.adj) * s2;
-      const int nblk = cg_grid(n);
-      sb.j[i] = SnrJob<T>{B.Rs, B.Dcg, ts[i]->scal(), ts[i]->iflag() + kActive, J.partial, n, nblk};
-      gmax = std::max(gmax, nblk);
-      sj.push_back(SumJob{J.partial, nblk, ts[i]->scal() + kD2, ts[i]->iflag() + kActive});
+      ctl.partial[i] = ts[i]->job()->partial;
+      ctl.np[i] = dj[i].np;
     }
-    hipLaunchKernelGGL(k_cg_snr<T>, dim3(gmax, nb), dim3(kT), 0, st_, sb);
-    PMS_LAUNCH_CHECK();
   }
-  sum_partials_batch(sj.data(), nb, st_);
   // exit rules of every job of the chunk (main.py:427-457)
-  CtlBatch cb2{};
-  cb2.n = nb;
-  for (int i = 0; i < nb; ++i) {
-    cb2.scal[i] = ts[i]->scal();
-    cb2.iflag[i] = ts[i]->iflag();
-    cb2.miter[i] = ts[i]->spec.prm.minor_loop_miter;
-  }
-  hipLaunchKernelGGL(k_minor_ctl, dim3(1), dim3(64), 0, st_, cb2);
+  hipLaunchKernelGGL(k_minor_ctl, dim3(nb), dim3(kT), 0, st_, ctl);
   PMS_LAUNCH_CHECK();
 }
 
