@@ -14,6 +14,17 @@ of the plain copy kernel of known size for the counter calibration)::
 
 FETCH_SIZE and WRITE_SIZE are in KiB per dispatch; each is scaled by (true bytes of the copy) / (counter
 of the copy), so the figures are bytes as the plain copy kernel counts them.
+
+Shader counters (``--pmc`` runs of the workload, each with its own counter set, beside its kernel trace)::
+
+    python prof_summary.py counters DIR/NAME_kernel_trace.csv PMC1.csv PMC2.csv ... --kernel REGEX ... \
+        --title TITLE --commands FILE > SUMMARY.md
+
+On gfx950 the SQ cycle counters (SQ_WAVE_CYCLES, SQ_BUSY_CYCLES, SQ_ACTIVE_INST_*, SQ_WAIT_*) count
+quad-cycles, and GRBM_GUI_ACTIVE counts cycles summed over the 8 XCDs: the kernel's cycles under the
+counters are GRBM_GUI_ACTIVE / 8, the mean resident waves per SIMD are 4 SQ_WAVE_CYCLES / (cycles x
+1024 SIMDs), and the share of SIMD cycles issuing vector instructions is 4 SQ_ACTIVE_INST_VALU /
+(cycles x 1024).
 """
 
 import argparse
@@ -146,6 +157,50 @@ def traffic(args):
             json.dump(doc, f, indent=1)
 
 
+SIMDS = 1024  # 256 CUs x 4 SIMDs
+VALU_ISSUE = SIMDS * 2.4e9 / 4  # wave64 vector instructions per second without packed FP32
+
+
+def counters(args):
+    dur = collections.defaultdict(list)
+    with open(args.kernel_trace) as f:
+        for r in csv.DictReader(f):
+            dur[short_name(r["Kernel_Name"])].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1000.0)
+    vals = collections.defaultdict(lambda: collections.defaultdict(list))
+    res = {}
+    for path in args.pmc:
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                n = short_name(r["Kernel_Name"])
+                vals[n][r["Counter_Name"]].append(float(r["Counter_Value"]))
+                res[n] = (r["Workgroup_Size"], r["VGPR_Count"], r["Accum_VGPR_Count"], r["SGPR_Count"])
+    names = [n for n in vals if any(re.search(k, n) for k in args.kernel)]
+    names.sort(key=lambda n: -sum(dur.get(n, [0.0])) )
+    out = ["# Shader counters: %s" % args.title, "", "Commands (repo root, on an MI355X):", "", "```",
+           open(args.commands).read().rstrip(), "```", "",
+           "Per kernel: the trace's average duration, the counters' average per dispatch, and what they imply "
+           "(SQ cycle counters in quad-cycles; the kernel's cycles under the counters = GRBM_GUI_ACTIVE / 8).", "",
+           "| kernel | block, VGPR | avg us (trace) | VALU wave-instr | VALU floor us | SIMD VALU busy | "
+           "waves/SIMD | LDS instr | LDS-wait share | bank-conflict cycles / CU | any-wait share |",
+           "|---|---|---|---|---|---|---|---|---|---|---|"]
+
+    def avg(n, c):
+        v = vals[n].get(c)
+        return sum(v) / len(v) if v else float("nan")
+
+    for n in names:
+        d = dur.get(n, [float("nan")])
+        us = sum(d) / len(d)
+        cyc = avg(n, "GRBM_GUI_ACTIVE") / 8.0
+        valu = avg(n, "SQ_INSTS_VALU")
+        wave = avg(n, "SQ_WAVE_CYCLES")
+        out.append("| %s | %s, %s | %.1f | %.3g | %.1f | %.2f | %.2f | %.3g | %.2f | %.0f | %.2f |" % (
+            n, res[n][0], res[n][1], us, valu, valu / VALU_ISSUE * 1e6, 4 * avg(n, "SQ_ACTIVE_INST_VALU") / (cyc * SIMDS),
+            4 * wave / (cyc * SIMDS), avg(n, "SQ_INSTS_LDS"), avg(n, "SQ_WAIT_INST_LDS") / wave,
+            avg(n, "SQ_LDS_BANK_CONFLICT") / 256.0, avg(n, "SQ_WAIT_ANY") / wave))
+    print("\n".join(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -167,8 +222,14 @@ def main():
     r.add_argument("--commands", required=True)
     r.add_argument("--workload-key")
     r.add_argument("--json")
+    c = sub.add_parser("counters")
+    c.add_argument("kernel_trace")
+    c.add_argument("pmc", nargs="+")
+    c.add_argument("--kernel", action="append", required=True, help="REGEX on the kernel's short name")
+    c.add_argument("--title", required=True)
+    c.add_argument("--commands", required=True)
     args = ap.parse_args()
-    trace(args) if args.cmd == "trace" else traffic(args)
+    {"trace": trace, "traffic": traffic, "counters": counters}[args.cmd](args)
 
 
 if __name__ == "__main__":

@@ -90,8 +90,11 @@ struct ModeTraits<IUWT_RECOMP> {
 // Small planes (whole plane L2-resident): one thread per output pixel loads its full dilated
 // 9x9 (decomposition) or 5x5 neighbourhood at once, so a scale costs one memory latency instead
 // of a row walk.  Same arithmetic as k_iuwt_walk: row sums first, then the vertical combination.
+// Every load is issued before the first is used (at most 4 waves per SIMD, so the compiler may keep
+// all 81 values in registers instead of waiting row by row to save registers for occupancy that
+// these small grids never reach).
 template <int MODE>
-__global__ __launch_bounds__(kThreads) void k_iuwt_small(ScaleParams sp) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_iuwt_small(ScaleParams sp) {
   const ScaleJob a = sp.job[blockIdx.z];
   constexpr int H = ModeTraits<MODE>::H;
   constexpr int T = 2 * H + 1;
@@ -105,13 +108,17 @@ __global__ __launch_bounds__(kThreads) void k_iuwt_small(ScaleParams sp) {
   int cidx[T];
 #pragma unroll
   for (int b = 0; b < T; ++b) cidx[b] = fold_idx(col + (b - H) * d, w);
-  float ra[T], rb[T], xc = 0.f;
+  float vv[T][T];
 #pragma unroll
   for (int i = 0; i < T; ++i) {
     const float* src = a.in + (size_t)fold_idx(row + (i - H) * d, h) * w;
-    float v[T];
 #pragma unroll
-    for (int b = 0; b < T; ++b) v[b] = src[cidx[b]];
+    for (int b = 0; b < T; ++b) vv[i][b] = src[cidx[b]];
+  }
+  float ra[T], rb[T], xc = 0.f;
+#pragma unroll
+  for (int i = 0; i < T; ++i) {
+    const float* v = vv[i];
     float sa = 0.f, sb = 0.f;
 #pragma unroll
     for (int b = 0; b < 5; ++b) sa = fmaf(c_h5[b], v[b + H - 2], sa);
