@@ -145,10 +145,14 @@ static_assert(sizeof(SelStateT<uint32_t>) <= kSelBytes && sizeof(SelStateT<unsig
 constexpr int kBins = 2048;
 constexpr int kMaxPlanes = 16;  // plane stacks handled per call (scale counts up to 16)
 
+// Histogram of one digit of the keys whose higher digits equal the prefix found so far.  With
+// `above` set (the last digit), keys whose higher digits exceed the prefix are all larger than the
+// low median: the smallest of them goes to hi_key (atomic min), where the last select finds it if
+// the low median's bucket holds no larger key (the successor of even counts, without a pass of its own).
 template <class T, bool MASKED>
 __global__ __launch_bounds__(kT) void k_mad_hist(const T* __restrict__ planes, size_t plane, int w, Rect inc,
-                                                 Rect exc, int shift, int nbits, int prefix_shift,
-                                                 const SelStateT<typename MadKey<T>::K>* __restrict__ st,
+                                                 Rect exc, int shift, int nbits, int prefix_shift, int above,
+                                                 SelStateT<typename MadKey<T>::K>* __restrict__ st,
                                                  uint32_t* __restrict__ hist, const int* pred) {
   using K = typename MadKey<T>::K;
   if (pred_off(pred)) return;
@@ -159,9 +163,17 @@ __global__ __launch_bounds__(kT) void k_mad_hist(const T* __restrict__ planes, s
   const T* x = planes + (size_t)s * plane;
   const K pre = st[s].prefix;
   const K dmask = ((K)1 << nbits) - 1;
+  const K none = ~(K)0;
+  K best = none;
   auto add = [&](T v) {
     const K key = MadKey<T>::key(v);
-    if (prefix_shift < (int)(8 * sizeof(K)) && (key >> prefix_shift) != (pre >> prefix_shift)) return;
+    if (prefix_shift < (int)(8 * sizeof(K))) {
+      const K kp = key >> prefix_shift, pp = pre >> prefix_shift;
+      if (kp != pp) {
+        if (above && kp > pp && key < best) best = key;
+        return;
+      }
+    }
     atomicAdd(&h[(uint32_t)((key >> shift) & dmask)], 1u);
   };
   const size_t stride = (size_t)gridDim.x * kT;
@@ -198,6 +210,22 @@ __global__ __launch_bounds__(kT) void k_mad_hist(const T* __restrict__ planes, s
   uint32_t* gh = hist + (size_t)s * kBins;
   for (int i = threadIdx.x; i < kBins; i += kT)
     if (h[i]) atomicAdd(&gh[i], h[i]);
+  if (above) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const K v = __shfl_xor(best, o, 64);
+      best = v < best ? v : best;
+    }
+    // one atomic per block (per-wave atomics on the one address serialise at the L2)
+    __shared__ K wmin[kT / 64];
+    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int k = 1; k < kT / 64; ++k) best = wmin[k] < best ? wmin[k] : best;
+      if (best != none) atomicMin(&st[s].hi_key, best);
+    }
+  }
 }
 
 // one block per plane: locate the bucket holding the target rank, extend the prefix, clear the histogram
@@ -237,6 +265,8 @@ __global__ __launch_bounds__(kT) void k_mad_select(SelStateT<K>* st, uint32_t* h
   // the thread whose range of ranks [exc, inc) holds the target rank finishes the search (the
   // last thread also takes a rank beyond the total, as the serial scan did)
   const bool mine = (S0.rank >= exc && S0.rank < inc) || (threadIdx.x == kT - 1 && S0.rank >= inc);
+  __shared__ SelStateT<K> s_state;
+  __shared__ int s_bin;
   if (mine) {
     SelStateT<K> S = S0;
     uint32_t run = exc;
@@ -248,65 +278,37 @@ __global__ __launch_bounds__(kT) void k_mad_select(SelStateT<K>* st, uint32_t* h
     }
     S.rank -= run;
     S.prefix |= (K)b << shift;
-    if (last) {
-      S.eq = cnt[b];
-      S.hi_key = ~(K)0;
-    }
-    st[s] = S;
+    if (last) S.eq = cnt[b];
+    s_state = S;
+    s_bin = b;
   }
-}
-
-// min key strictly above the low median (even counts whose two middles differ)
-template <class T, bool MASKED>
-__global__ __launch_bounds__(kT) void k_mad_above(const T* __restrict__ planes, size_t plane, int w, Rect inc,
-                                                  Rect exc, SelStateT<typename MadKey<T>::K>* st, const int* pred) {
-  using K = typename MadKey<T>::K;
-  if (pred_off(pred)) return;
-  const int s = blockIdx.y;
-  const T* x = planes + (size_t)s * plane;
-  const K lo = st[s].prefix;
-  const K none = ~(K)0;
-  K best = none;
-  auto take = [&](T v) {
-    const K key = MadKey<T>::key(v);
-    if (key > lo && key < best) best = key;
-  };
-  const size_t stride = (size_t)gridDim.x * kT;
-  bool vec = false;
-  if constexpr (!MASKED && sizeof(T) == 4) vec = (reinterpret_cast<uintptr_t>(x) & 15) == 0;
-  if constexpr (sizeof(T) == 4) {
-    if (vec) {
-      const float4* x4 = reinterpret_cast<const float4*>(x);
-      const size_t n4 = plane >> 2;
-      for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n4; i += stride) {
-        const float4 a = x4[i];
-        take(a.x), take(a.y), take(a.z), take(a.w);
-      }
-      for (size_t j = 4 * n4 + blockIdx.x * (size_t)kT + threadIdx.x; j < plane; j += stride) take(x[j]);
-    }
+  __syncthreads();
+  if (!last) {
+    if (threadIdx.x == 0) st[s] = s_state;
+    return;
   }
-  if (!vec) {
-    for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < plane; i += stride) {
-      if (MASKED) {
-        const int r = (int)(i / w), c = (int)(i % w);
-        if (!in_noise_mask(r, c, inc, exc)) continue;
-      }
-      take(x[i]);
-    }
+  // the smallest key above the low median: the next non-empty bucket of this last digit (same
+  // higher digits), else the smallest key with larger higher digits (hi_key, from the histogram pass)
+  const int b = s_bin;
+  int nxt = kBins;
+  for (int k = 0; k < per; ++k) {
+    const int bb = threadIdx.x * per + k;
+    if (bb > b && bb < nb && cnt[bb] != 0u && bb < nxt) nxt = bb;
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
-    const K v = __shfl_xor(best, o, 64);
-    best = v < best ? v : best;
+    const int v = __shfl_xor(nxt, o, 64);
+    nxt = v < nxt ? v : nxt;
   }
-  // one atomic per block (per-wave atomics on the one address serialise at the L2)
-  __shared__ K wmin[kT / 64];
-  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = best;
+  __shared__ int s_nxt[kT / 64];
+  if (lane == 0) s_nxt[wid] = nxt;
   __syncthreads();
   if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 1; k < kT / 64; ++k) best = wmin[k] < best ? wmin[k] : best;
-    if (best != none) atomicMin(&st[s].hi_key, best);
+    for (int k = 1; k < kT / 64; ++k) nxt = s_nxt[k] < nxt ? s_nxt[k] : nxt;
+    SelStateT<K> S = s_state;
+    const K dm = (((K)1 << nbits) - 1) << shift;  // this digit's bits
+    if (nxt < nb) S.hi_key = (S.prefix & ~dm) | ((K)nxt << shift);
+    st[s] = S;
   }
 }
 
@@ -318,6 +320,7 @@ __global__ void k_mad_init(SelStateT<K>* st, int S, uint32_t n) {
   z.n = n;
   z.rank = (n - 1) / 2;
   z.prefix = 0;
+  z.hi_key = ~(K)0;  // lowered by the last histogram pass and the last select
   st[s] = z;
 }
 
@@ -916,8 +919,8 @@ void mad_work_init(void* work, int, hipStream_t st) {
 }
 
 // Exact median per plane: radix select over the key bits in digits of at most 11 bits, one
-// histogram pass and one select step per digit, then the successor of the low median for even
-// counts.
+// histogram pass and one select step per digit; for even counts the last pass and select also find
+// the successor of the low median.
 template <class T>
 static void mad_thresholds_t(const T* planes, int S, int h, int w, const NoiseMask& nm, double* thr, void* work,
                              const int* pred, hipStream_t st) {
@@ -943,25 +946,21 @@ static void mad_thresholds_t(const T* planes, int S, int h, int w, const NoiseMa
     top -= bits[pass];
     shifts[pass] = top;
   }
+  // even counts also need the smallest key above the low median: the last histogram pass and the
+  // last select find it on the way
+  const int above = (nm.count & 1) == 0 ? 1 : 0;
   for (int pass = 0; pass < kPasses; ++pass) {
     const int pshift = (pass == 0) ? (int)(8 * sizeof(K)) : shifts[pass - 1];
+    const int ab = pass == kPasses - 1 ? above : 0;
     if (nm.masked)
       hipLaunchKernelGGL((k_mad_hist<T, true>), dim3(gx, S), dim3(kT), 0, st, planes, plane, w, inc, exc,
-                         shifts[pass], bits[pass], pshift, (const SelStateT<K>*)sst, hist, pred);
+                         shifts[pass], bits[pass], pshift, ab, sst, hist, pred);
     else
       hipLaunchKernelGGL((k_mad_hist<T, false>), dim3(gx, S), dim3(kT), 0, st, planes, plane, w, inc, exc,
-                         shifts[pass], bits[pass], pshift, (const SelStateT<K>*)sst, hist, pred);
+                         shifts[pass], bits[pass], pshift, ab, sst, hist, pred);
     PMS_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_mad_select<K>, dim3(S), dim3(kT), 0, st, sst, hist, shifts[pass], bits[pass],
                        pass == kPasses - 1, pred);
-    PMS_LAUNCH_CHECK();
-  }
-  if ((nm.count & 1) == 0) {
-    if (nm.masked)
-      hipLaunchKernelGGL((k_mad_above<T, true>), dim3(gx, S), dim3(kT), 0, st, planes, plane, w, inc, exc, sst, pred);
-    else
-      hipLaunchKernelGGL((k_mad_above<T, false>), dim3(gx, S), dim3(kT), 0, st, planes, plane, w, inc, exc, sst,
-                         pred);
     PMS_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(k_mad_finish<T>, dim3(ceil_div(S, 64)), dim3(64), 0, st, (const SelStateT<K>*)sst, S, thr, pred);
