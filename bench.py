@@ -409,9 +409,9 @@ def main():
                                                             "--no-cpu), print its line, exit")
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--max-batch", type=int, default=32, help="jobs per lockstep batch (batched/served forms)")
-    ap.add_argument("--calibrate-runs64", action="store_true",
-                    help="only run a copy that reads 64-byte runs (every other 64 bytes of a 512 MiB buffer) "
-                         "5 times: the HBM counter calibration for short runs")
+    ap.add_argument("--calibrate-runs", type=int, nargs="?", const=64, metavar="BYTES",
+                    help="only run a copy that reads runs of BYTES (default 64) bytes, every other run of a "
+                         "512 MiB buffer, 5 times: the HBM counter calibration for short runs")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB float32 copy kernel 5 times (HBM counter calibration)")
     ap.add_argument("--precision", nargs="?", const="", metavar="OUT_NPZ",
@@ -434,24 +434,27 @@ def main():
     from pymoresane_amd._lib import DeviceArray, Event, check
 
     lib = _lib.lib()
-    if args.calibrate_runs64:
-        # the HIP runtime's pitched copy: the first 64 bytes of every 128-byte row of a 512 MiB buffer
-        # (one 64-byte run per row) into a dense 256 MiB buffer, 5 times
+    if args.calibrate_runs:
+        # the HIP runtime's pitched copy: the first RUN bytes of every 2*RUN-byte row of a 512 MiB buffer
+        # (one RUN-byte run per row) into a dense 256 MiB buffer, 5 times
+        run = args.calibrate_runs
+        if run < 4 or run % 4 or (256 << 20) % run:
+            raise SystemExit("--calibrate-runs takes a multiple of 4 bytes that divides 256 MiB")
         hip = ctypes.CDLL("libamdhip64.so")
         hip.hipMemcpy2D.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
                                     ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int]
         hip.hipMemcpy2D.restype = ctypes.c_int
-        rows = (512 << 20) // 128
-        a, b2 = DeviceArray((rows * 32,)), DeviceArray((rows * 16,))
+        rows = (256 << 20) // run
+        a, b2 = DeviceArray((rows * run // 2,)), DeviceArray((rows * run // 4,))
         a.fill(1.0)
         check(lib.pms_device_synchronize())
         t0 = time.perf_counter()
         for _ in range(5):
-            if hip.hipMemcpy2D(b2.ptr, 64, a.ptr, 128, 64, rows, 3) != 0:  # 3: device to device
+            if hip.hipMemcpy2D(b2.ptr, run, a.ptr, 2 * run, run, rows, 3) != 0:  # 3: device to device
                 raise RuntimeError("hipMemcpy2D failed")
         check(lib.pms_device_synchronize())
         ms = (time.perf_counter() - t0) * 1e3 / 5
-        print(json.dumps({"read_bytes_in_64B_runs": rows * 64, "write_bytes": rows * 64, "ms": round(ms, 4)}))
+        print(json.dumps({"run_bytes": run, "read_bytes": rows * run, "write_bytes": rows * run, "ms": round(ms, 4)}))
         return
     if args.calibrate_copy:
         count = 64 << 20

@@ -13,7 +13,11 @@ of the plain copy kernel of known size for the counter calibration)::
         [--workload-key JSON] [--json OUT.json] --commands FILE > SUMMARY.md
 
 FETCH_SIZE and WRITE_SIZE are in KiB per dispatch; each is scaled by (true bytes of the copy) / (counter
-of the copy), so the figures are bytes as the plain copy kernel counts them.
+of the copy), so the figures are bytes as the plain copy kernel counts them.  On gfx950 FETCH_SIZE
+counts a 128-byte read request as 64 bytes (a copy of 128-byte or longer runs reads 0.5x, one of 32- or
+64-byte runs 1.0x the true bytes), so one scale factor fits only kernels with one request size; with
+``--fetch-requests`` the fetch runs are ``--pmc TCC_EA0_RDREQ_32B TCC_EA0_RDREQ_64B TCC_EA0_RDREQ_128B
+TCC_EA0_RDREQ`` instead and each request counts its own size.
 
 Shader counters (``--pmc`` runs of the workload, each with its own counter set, beside its kernel trace)::
 
@@ -87,16 +91,39 @@ def trace(args):
     print("\n".join(out))
 
 
+def load_request_bytes(path):
+    """Read bytes per dispatch from a run of TCC_EA0_RDREQ_32B/_64B/_128B: 32, 64 and 128 bytes per
+    request (FETCH_SIZE's expression counts a 128-byte request as 64 bytes on gfx950), in KiB."""
+    per = collections.defaultdict(dict)
+    meta = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            d = r["Dispatch_Id"]
+            per[d][r["Counter_Name"]] = float(r["Counter_Value"])
+            meta[d] = (short_name(r["Kernel_Name"]), int(r["Grid_Size"]))
+    rows = []
+    for d, c in per.items():
+        b = 32 * c.get("TCC_EA0_RDREQ_32B", 0.0) + 64 * c.get("TCC_EA0_RDREQ_64B", 0.0) + \
+            128 * c.get("TCC_EA0_RDREQ_128B", 0.0)
+        rows.append((meta[d][0], meta[d][1], b / 1024.0))
+    return rows
+
+
 def traffic(args):
-    fetch = load_counters(args.fetch, "FETCH_SIZE")
+    if args.fetch_requests:
+        fetch = load_request_bytes(args.fetch)
+        cf = [v for n, g, v in load_request_bytes(args.copy_fetch) if n.startswith("k_copy")]
+    else:
+        fetch = load_counters(args.fetch, "FETCH_SIZE")
+        cf = [v for n, g, v in load_counters(args.copy_fetch, "FETCH_SIZE") if n.startswith("k_copy")]
     write = load_counters(args.write, "WRITE_SIZE")
-    cf = [v for n, g, v in load_counters(args.copy_fetch, "FETCH_SIZE") if n.startswith("k_copy")]
     cw = [v for n, g, v in load_counters(args.copy_write, "WRITE_SIZE") if n.startswith("k_copy")]
     if not cf or not cw:
         sys.exit("no k_copy dispatches in the calibration runs")
     kib_f, kib_w = sum(cf) / len(cf), sum(cw) / len(cw)
     f_scale = args.copy_bytes / (kib_f * 1024.0)
     w_scale = args.copy_bytes / (kib_w * 1024.0)
+    fetch_name = "read bytes from the request counts" if args.fetch_requests else "FETCH_SIZE"
 
     def table(rows):
         t = collections.defaultdict(list)
@@ -108,10 +135,11 @@ def traffic(args):
     out = ["# HBM traffic per launch: %s" % args.title, "",
            "Commands (repo root, on an MI355X), one counter per pass:", "",
            "```", open(args.commands).read().rstrip(), "```", "",
-           "Calibration: the plain copy kernel (k_copy) moving %d bytes each way reads FETCH_SIZE = %.1f KiB and "
+           "Calibration: the plain copy kernel (k_copy) moving %d bytes each way reads %s = %.1f KiB and "
            "WRITE_SIZE = %.1f KiB per dispatch, so the scale factors to true bytes are **fetch x%.3f** and "
-           "**write x%.3f**." % (args.copy_bytes, kib_f, kib_w, f_scale, w_scale), "",
-           "| kernel | grid | dispatches | FETCH KiB (raw avg) | WRITE KiB (raw avg) | traffic MB (calibrated) |",
+           "**write x%.3f**." % (args.copy_bytes, fetch_name, kib_f, kib_w, f_scale, w_scale), "",
+           "| kernel | grid | dispatches | %s KiB (raw avg) | WRITE KiB (raw avg) | traffic MB (calibrated) |"
+           % ("read" if args.fetch_requests else "FETCH"),
            "|---|---|---|---|---|---|"]
     per_kernel = {}
     for key in sorted(set(tf) | set(tw), key=lambda k: -sum(tf.get(k, [0])) - sum(tw.get(k, [0]))):
@@ -151,7 +179,8 @@ def traffic(args):
     print("\n".join(out))
     if args.json:
         doc = {"workload": json.loads(args.workload_key) if args.workload_key else None,
-               "calibration": {"copy_bytes": args.copy_bytes, "fetch_scale": f_scale, "write_scale": w_scale},
+               "calibration": {"copy_bytes": args.copy_bytes, "fetch_scale": f_scale, "write_scale": w_scale,
+                               "fetch_counter": "TCC_EA0_RDREQ_32B/64B/128B" if args.fetch_requests else "FETCH_SIZE"},
                "ops": ops}
         with open(args.json, "w") as f:
             json.dump(doc, f, indent=1)
@@ -216,6 +245,8 @@ def main():
     for a in ("fetch", "write", "copy_fetch", "copy_write"):
         r.add_argument(a)
     r.add_argument("--copy-bytes", type=float, required=True)
+    r.add_argument("--fetch-requests", action="store_true",
+                   help="FETCH and COPY_FETCH are runs of TCC_EA0_RDREQ_32B/_64B/_128B instead of FETCH_SIZE")
     r.add_argument("--op", action="append", default=[])
     r.add_argument("--bench")
     r.add_argument("--title", required=True)
