@@ -603,6 +603,265 @@ __global__ __launch_bounds__(NT) void k_iuwt_recomp2(PairParams sp) {
   }
 }
 
+// ---- two decomposition scales per launch ----------------------------------------------------
+// Operands of one job of a two-scale decomposition: from C_s the detail planes w_s and w_{s+1} and
+// the smoothed C_{s+2}; C_{s+1} = h_s C_s exists only as LDS rows.
+struct Dec2Job {
+  const float* in;       // C_s
+  float* out_c;          // C_{s+2} (null: not kept)
+  float* out_w0;         // w_s
+  float* out_w1;         // w_{s+1}
+  const uint8_t* mask0;  // masks of w_s and w_{s+1} (null = none)
+  const uint8_t* mask1;
+  const int* pred;
+};
+struct Dec2Params {
+  Dec2Job job[kMaxBatch];
+  int h, w, d, q_log2, tj, k_rows, n_cg, n_tb, n_seg;
+};
+
+// A two-scale walk has at most this many steps: its per-step tables use two registers (steps
+// 0..63 and 64..127), so the 21 lead-in and lag steps are a small share of a walk.
+constexpr int kDec2MaxSteps = 128;
+
+// Two-scale decomposition walk over the comb/phase frame of scale s (dilation d = 2^s), where scale
+// s+1 is the same filter at tooth and row spacing 2.  Stage 1 filters the staged C_s row (halo 10
+// teeth) into C_{s+1} for one C_{s+1} lane per thread (teeth -8 .. TJ+7, staged in an LDS row) and
+// into w_s = C_s - H9 C_s for the output lane; stage 2 filters the C_{s+1} row of the previous step
+// at spacing 2 into C_{s+2} and w_{s+1} = C_{s+1} - H9 C_{s+1}.  Input row i gives C_{s+1} row
+// i-2 and w_s row i-4 in stage 1; C_{s+1} row i-3 gives C_{s+2} row i-7 and w_{s+1} row i-11 in
+// stage 2, so a walk of n output rows reads n+20 input rows in n+21 steps, with one barrier per
+// step (it publishes the step's C_s row and the previous step's C_{s+1} row; both are double
+// buffered).  Every sum has the one-scale walk's operands in its order, so the planes are
+// bit-identical to two one-scale launches: C_{s+1} lanes and rows beyond the border are computed at
+// the unfolded position, which on the symmetric extension equals the folded one when a mirrored lane
+// or row accumulates its taps from the other end (as in the two-scale recomposition).
+template <int FL, int K, int ML, int NT>
+__global__ __launch_bounds__(NT) void k_iuwt_dec2(Dec2Params sp) {
+  constexpr bool OUTC = (FL & F_OUTC) != 0;
+  constexpr bool MASK = (FL & F_MASK) != 0;
+  constexpr int HT = 8, HC = 10;  // C_{s+1} lanes and staged C_s teeth beyond the output teeth
+  constexpr int LAG = 10;         // input rows before the first output row
+  static_assert(K % 2 == 0, "group length must be even (LDS double buffer parity)");
+  const Dec2Job a = sp.job[blockIdx.y];
+  if (a.pred && *a.pred == 0) return;
+
+  // LDS: two C_s rows of ML*NT slots and two C_{s+1} rows of NT slots (every lane writes its slot)
+  extern __shared__ float lds[];
+  float* cbuf0 = lds;
+  float* cbuf1 = lds + ML * NT;
+  float* tbuf0 = lds + 2 * ML * NT;
+  float* tbuf1 = tbuf0 + NT;
+  const int Q = 1 << sp.q_log2;
+  const int TJ = sp.tj;
+  const int c_elems = (TJ + 2 * HC) * Q;
+  const int t_elems = (TJ + 2 * HT) * Q;
+
+  const unsigned nblk = gridDim.x;
+  unsigned b = xcd_remap(blockIdx.x, nblk);
+  const int seg = b % sp.n_seg;
+  b /= sp.n_seg;
+  const int tb = b % sp.n_tb;
+  b /= sp.n_tb;
+  const int cg = b % sp.n_cg;
+  const int p = b / sp.n_cg;  // row phase
+  const int d = sp.d;
+  const int h = sp.h, w = sp.w;
+
+  const int t = threadIdx.x;
+  const int q = t & (Q - 1);
+  const int jt = (t >> sp.q_log2) - HT;  // tooth of this thread's C_{s+1} lane and output lane
+  const bool t_live = t < t_elems;
+  const bool o_live = jt >= 0 && jt < TJ;
+  const int col = cg * Q + q + (tb * TJ + jt) * d;  // natural column of the lane (possibly beyond a border)
+  const bool col_ok = o_live && col < w;
+  const unsigned col_off = col_ok ? 4u * (unsigned)col : 0u;
+  const bool tl_mir = t_live && fold_mirrors(col, w);
+  const int c_lds = t_live ? t + 2 * Q : 2 * Q;  // C_s slot of the lane's tooth (dead lanes: valid slots)
+  const int o_lds = o_live ? t : HT * Q;         // the output lane's C_{s+1} slot; its C_s slot is o_lds + 2Q
+
+  const int mr = (h - p + d - 1) / d;
+  const int o_begin = seg * sp.k_rows;
+  const int o_end = min(o_begin + sp.k_rows, mr);
+  const int n_out = o_end - o_begin;
+  const int i_begin = o_begin - LAG;
+  const int n_in = n_out + 2 * LAG;
+  const int n_steps = n_in + 1;  // <= kDec2MaxSteps (plan)
+  const int n_groups = (n_steps + K - 1) / K;
+
+  // per-step row byte offsets, step = lane (and lane + 64): input rows clamped to the walk (the
+  // prefetch past the end re-reads the last row), output rows clamped to a real row for steps that
+  // produce none
+  const int lane = t & 63;
+  auto in_row = [&](int s) { return fold_idx(p + (i_begin + min(s, n_in - 1)) * d, h) * w * 4; };
+  auto out_row = [&](int s, int lag) { return (p + min(max(i_begin + s - lag, o_begin), o_end - 1) * d) * w * 4; };
+  const int tin0 = in_row(lane), tin1 = in_row(lane + 64);
+  const int tw0a = out_row(lane, 4), tw0b = out_row(lane + 64, 4);
+  const int tca = out_row(lane, 7), tcb = out_row(lane + 64, 7);
+  const int tw1a = out_row(lane, 11), tw1b = out_row(lane + 64, 11);
+  // the C_{s+1} row of step s (input row - 2) lies beyond a border: its vertical taps run reversed
+  auto mir_of = [&](int s) { return fold_mirrors(p + (i_begin + s - 2) * d, h); };
+  const unsigned long long mira = __ballot(mir_of(lane));
+  const unsigned long long mirb = __ballot(mir_of(lane + 64));
+  // both table registers are read and the step's one selected, without a branch
+  auto tab = [](int lo, int hi, int s) {
+    const unsigned a = (unsigned)__builtin_amdgcn_readlane(lo, s & 63);
+    const unsigned b = (unsigned)__builtin_amdgcn_readlane(hi, s & 63);
+    return s < 64 ? a : b;
+  };
+  auto mirrored = [&](int s) { return (((s < 64 ? mira : mirb) >> (s & 63)) & 1ull) != 0; };
+
+  unsigned ld_off[ML];
+#pragma unroll
+  for (int m = 0; m < ML; ++m) {
+    const int e = t + m * NT;
+    const int jj = e >> sp.q_log2;
+    const int qq = e & (Q - 1);
+    ld_off[m] = e < c_elems ? 4u * (unsigned)fold_idx(cg * Q + qq + (tb * TJ + jj - HC) * d, w) : 0u;
+  }
+  const unsigned plane = (unsigned)h * (unsigned)w;
+  const auto r_in = plane_rsrc(a.in, 4u * plane);
+  const auto r_m0 = plane_rsrc(a.mask0, plane);
+  const auto r_m1 = plane_rsrc(a.mask1, plane);
+
+  struct Group {
+    float row[K][ML];
+    int m0[K];
+    int m1[K];
+  };
+  auto load_group = [&](Group& G, int g) {
+#pragma unroll
+    for (int u = 0; u < K; ++u) {
+      const int s = g * K + u;
+      const unsigned s_in = tab(tin0, tin1, s);
+#pragma unroll
+      for (int m = 0; m < ML; ++m) G.row[u][m] = buf_f32(r_in, ld_off[m], s_in);
+      if constexpr (MASK) {
+        G.m0[u] = buf_u8(r_m0, col_off >> 2, tab(tw0a, tw0b, s) >> 2);
+        G.m1[u] = buf_u8(r_m1, col_off >> 2, tab(tw1a, tw1b, s) >> 2);
+      }
+    }
+  };
+  auto put_row = [&](float* buf, const float* row) {
+#pragma unroll
+    for (int m = 0; m < ML; ++m) buf[t + m * NT] = row[m];
+  };
+
+  float va[5], vb[9], vx[5], v2a[9], v2b[17], v2x[9];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) va[k] = vx[k] = 0.f;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) vb[k] = v2a[k] = v2x[k] = 0.f;
+#pragma unroll
+  for (int k = 0; k < 17; ++k) v2b[k] = 0.f;
+
+  auto dst = [&](float* base, unsigned row_off, bool out) {
+    return out ? reinterpret_cast<float*>(reinterpret_cast<char*>(base) + row_off + col_off) : g_walk_sink + t;
+  };
+
+  auto step = [&](const Group& G, const Group& Gn, int u, int s) {
+    const float* ccur = (u & 1) ? cbuf1 : cbuf0;
+    float* cnext = (u & 1) ? cbuf0 : cbuf1;
+    float* tcur = (u & 1) ? tbuf1 : tbuf0;
+    const float* tprev = (u & 1) ? tbuf0 : tbuf1;
+    __syncthreads();
+    // stage 1, scale s (spacing 1)
+    {
+      float x[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) x[k] = ccur[c_lds + (k - 2) * Q];
+      // both accumulation orders, selected without a branch, so the two stages form one block of
+      // independent work for the scheduler
+      float ha = 0.f, hm = 0.f;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        ha = fmaf(c_h5[k], x[k], ha);
+        hm = fmaf(c_h5[k], x[4 - k], hm);
+      }
+      ha = tl_mir ? hm : ha;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) va[k] = va[k + 1];
+      va[4] = ha;
+      float c = 0.f, cm = 0.f;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        c = fmaf(c_h5[k], va[k], c);
+        cm = fmaf(c_h5[k], va[4 - k], cm);
+      }
+      tcur[t] = mirrored(s) ? cm : c;  // C_{s+1} of input row - 2
+      float y[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) y[k] = ccur[o_lds + (k - 2) * Q];
+      float hb = 0.f;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) hb = fmaf(c_h9[k], y[k], hb);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) vb[k] = vb[k + 1];
+      vb[8] = hb;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) vx[k] = vx[k + 1];
+      vx[4] = y[4];
+      float c1 = 0.f;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) c1 = fmaf(c_h9[k], vb[k], c1);
+      float wv = vx[0] - c1;  // w_s of input row - 4
+      if constexpr (MASK) wv = G.m0[u] == 0 ? 0.f : wv;
+      const bool out = (unsigned)(s - 2 * LAG + 6) < (unsigned)n_out && col_ok;
+      __builtin_nontemporal_store(wv, dst(a.out_w0, tab(tw0a, tw0b, s), out));
+    }
+    // stage 2, scale s+1 (spacing 2), on the C_{s+1} row of the previous step (input row - 3)
+    {
+      float z[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) z[k] = tprev[o_lds + 2 * (k - 4) * Q];
+      float ha = 0.f, hb = 0.f;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) ha = fmaf(c_h5[k], z[k + 2], ha);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) hb = fmaf(c_h9[k], z[k], hb);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) v2a[k] = v2a[k + 1];
+      v2a[8] = ha;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) v2b[k] = v2b[k + 1];
+      v2b[16] = hb;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) v2x[k] = v2x[k + 1];
+      v2x[8] = z[4];
+      if constexpr (OUTC) {
+        float c = 0.f;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) c = fmaf(c_h5[k], v2a[2 * k], c);
+        const bool out = (unsigned)(s - 2 * LAG + 3) < (unsigned)n_out && col_ok;
+        *dst(a.out_c, tab(tca, tcb, s), out) = c;  // C_{s+2} of input row - 7
+      }
+      float c1 = 0.f;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) c1 = fmaf(c_h9[k], v2b[2 * k], c1);
+      float wv = v2x[0] - c1;  // w_{s+1} of input row - 11
+      if constexpr (MASK) wv = G.m1[u] == 0 ? 0.f : wv;
+      const bool out = (unsigned)(s - 2 * LAG - 1) < (unsigned)n_out && col_ok;
+      __builtin_nontemporal_store(wv, dst(a.out_w1, tab(tw1a, tw1b, s), out));
+    }
+    put_row(cnext, u + 1 < K ? G.row[u + 1] : Gn.row[0]);
+  };
+
+  tbuf0[t] = 0.f;
+  tbuf1[t] = 0.f;
+  Group GA, GB;
+  load_group(GA, 0);
+  put_row(cbuf0, GA.row[0]);
+  for (int g = 0; g < n_groups; g += 2) {
+    load_group(GB, g + 1);
+#pragma unroll
+    for (int u = 0; u < K; ++u) step(GA, GB, u, g * K + u);
+    load_group(GA, g + 2);
+    if (g + 1 >= n_groups) break;
+#pragma unroll
+    for (int u = 0; u < K; ++u) step(GB, GA, u, (g + 1) * K + u);
+  }
+}
+
 struct Plan {
   int q_log2, tj, k_rows, n_cg, n_tb, n_seg, blocks, lds_bytes;
   int nt = kThreads;  // threads per block
@@ -619,7 +878,7 @@ struct IuwtTuning {
   long small_pixels;  // planes up to this many pixels use the direct-stencil kernel
   int pair_min_rows;  // two-scale recomposition needs this many rows per row phase of the finer scale
   IuwtTuning()
-      : qmax(env_int("PMS_IUWT_QMAX", 16)),
+      : qmax(env_int("PMS_IUWT_QMAX", 32)),
         krows(env_int("PMS_IUWT_KROWS", 0)),
         target_blocks(env_int("PMS_IUWT_BLOCKS", 2048)),
         small_pixels(env_int("PMS_IUWT_SMALL", 256 * 256)),
@@ -633,8 +892,14 @@ const IuwtTuning& tuning() {
 // Launch parameters an autotuned call runs with (set for the duration of the call on its thread).
 struct TuneParams {
   int target_blocks;  // workgroups a one-scale walk aims for
-  int pair;           // recomposition pairing (0 off, 1 from the coarsest, 2 from the finest)
+  int pair;           // recomposition pairing (0 off, 1 from the coarsest, 2 from the finest);
+                      // decomposition: two-scale launches (1) or one scale per launch (0)
 };
+
+// One decomposition scale per launch unless PMS_IUWT_DEC2=1 asks for two-scale launches where they
+// apply: those give the same planes and move fewer bytes, but measured slower (their two dependent
+// filter stages per step leave the walk latency-bound rather than bandwidth-bound).
+constexpr int kDec2Default = 0;
 thread_local const TuneParams* t_active = nullptr;
 
 int plan_target_blocks() { return t_active ? t_active->target_blocks : tuning().target_blocks; }
@@ -812,6 +1077,96 @@ void recomp2_batch(const PairJob* jobs, int count, int h, int w, int s, const Pl
   }
 }
 
+// Launch shape of a two-scale decomposition at scales (s, s+1), or false where it does not apply:
+// planes for the row walks, d = 2^s with 20 d within both sides (every staged or intermediate
+// index folds at most once), and chunks of at most 8 phases (with 16 the 16 halo teeth of the
+// C_{s+1} lanes would take half of a block's lanes).
+bool dec2_plan(int h, int w, int s, Plan* out) {
+  if ((long)h * w <= tuning().small_pixels || (long)h * w >= (1L << 29)) return false;
+  const int d = 1 << s;
+  if (d > 8 || 20L * d > h || 20L * d > w) return false;
+  const int q = d;
+  const int nt = q <= 2 ? kThreads : 2 * kThreads;
+  const int teeth = ceil_div(w, d);
+  const int tj = std::min(nt / q - 16, teeth);
+  if (tj < 8) return false;
+  Plan pl;
+  pl.nt = nt;
+  pl.q_log2 = ilog2(q);
+  pl.tj = tj;
+  const int mr = ceil_div(h, d);
+  const int strips = d * ceil_div(d, q) * ceil_div(teeth, tj);
+  int k = tuning().krows;
+  if (k <= 0) {
+    const long target = plan_target_blocks();
+    long segs = (target + strips - 1) / strips;
+    if (segs < 1) segs = 1;
+    k = ceil_div(mr, (int)segs);
+    if (k < 40) k = 40;
+  }
+  if (k > kDec2MaxSteps - 21) k = kDec2MaxSteps - 21;  // a walk's steps fit two table registers
+  if (k > mr) k = mr;
+  pl.k_rows = k;
+  pl.n_cg = ceil_div(d, q);
+  pl.n_tb = ceil_div(teeth, tj);
+  pl.n_seg = ceil_div(mr, k);
+  pl.blocks = d * pl.n_cg * pl.n_tb * pl.n_seg;
+  pl.lds_bytes = (2 * 2 * nt + 2 * nt) * (int)sizeof(float);  // (tj + 20) q <= 2 nt C_s slots per row
+  *out = pl;
+  return true;
+}
+
+template <int FL, int NT>
+void launch_dec2_nt(Dec2Params& sp, int count, const Plan& pl, hipStream_t st) {
+  PMS_REQUIRE(((pl.tj + 20) << pl.q_log2) <= 2 * NT && ((pl.tj + 16) << pl.q_log2) <= NT,
+              "two-scale decomposition plan exceeds its block");
+  hipLaunchKernelGGL((k_iuwt_dec2<FL, kWalkGroup, 2, NT>), dim3(pl.blocks, count), dim3(NT), pl.lds_bytes, st, sp);
+  PMS_LAUNCH_CHECK();
+}
+
+template <int FL>
+void launch_dec2_fl(Dec2Params& sp, int count, const Plan& pl, hipStream_t st) {
+  if (pl.nt == 2 * kThreads)
+    launch_dec2_nt<FL, 2 * kThreads>(sp, count, pl, st);
+  else
+    launch_dec2_nt<FL, kThreads>(sp, count, pl, st);
+}
+
+// jobs[0..count) share shape and scale pair; jobs with and without a mask or a kept C_{s+2} go to
+// separate launches
+void dec2_batch(const Dec2Job* jobs, int count, int h, int w, int s, const Plan& pl, hipStream_t st) {
+  Dec2Params sp{};
+  sp.h = h;
+  sp.w = w;
+  sp.d = 1 << s;
+  sp.q_log2 = pl.q_log2;
+  sp.tj = pl.tj;
+  sp.k_rows = pl.k_rows;
+  sp.n_cg = pl.n_cg;
+  sp.n_tb = pl.n_tb;
+  sp.n_seg = pl.n_seg;
+  for (int kind = 0; kind < 4; ++kind) {
+    const bool masked = (kind & 1) != 0, keep_c = (kind & 2) != 0;
+    int nb = 0;
+    auto flush = [&]() {
+      if (nb == 0) return;
+      switch (kind) {
+        case 0: launch_dec2_fl<F_OUTW>(sp, nb, pl, st); break;
+        case 1: launch_dec2_fl<F_OUTW | F_MASK>(sp, nb, pl, st); break;
+        case 2: launch_dec2_fl<F_OUTW | F_OUTC>(sp, nb, pl, st); break;
+        default: launch_dec2_fl<F_OUTW | F_OUTC | F_MASK>(sp, nb, pl, st); break;
+      }
+      nb = 0;
+    };
+    for (int k = 0; k < count; ++k) {
+      if ((jobs[k].mask0 != nullptr) != masked || (jobs[k].out_c != nullptr) != keep_c) continue;
+      sp.job[nb++] = jobs[k];
+      if (nb == kMaxBatch) flush();
+    }
+    flush();
+  }
+}
+
 int walk_flags(const IuwtScaleCall& c) {
   if (c.mode == IUWT_DECOMP)
     return (c.out_c ? F_OUTC : 0) | (c.out_w ? F_OUTW : 0) | (c.mask ? F_MASK : 0) | (c.ref ? F_REF : 0);
@@ -944,6 +1299,7 @@ static void decompose_batch_impl(IuwtDecompJob* jobs, int count, int h, int w, h
   struct State {
     const float* cur;
     int slot;
+    int done;  // scales below this one are decomposed
   };
   std::vector<State> state(count);
   int smax = 0;
@@ -951,18 +1307,27 @@ static void decompose_batch_impl(IuwtDecompJob* jobs, int count, int h, int w, h
     IuwtDecompJob& j = jobs[k];
     PMS_REQUIRE(j.scale_adjust >= 0 && j.scale_count > j.scale_adjust, "need scale_count > scale_adjust >= 0");
     check_iuwt_scale(h, w, j.scale_count - 1);
-    state[k] = {j.in, 0};
+    state[k] = {j.in, 0, 0};
     j.np = 0;
     smax = std::max(smax, j.scale_count);
   }
+  // detail scales in pairs from the finest up where a two-scale launch applies (1), or one scale per
+  // launch (0): the autotuner's choice, else PMS_IUWT_DEC2 (the planes are identical either way)
+  const int fuse = t_active ? t_active->pair : env_int("PMS_IUWT_DEC2", kDec2Default);
   std::vector<IuwtScaleCall> calls;
   std::vector<int> who;
+  std::vector<Dec2Job> pairs;
+  std::vector<int> pwho;
   calls.reserve(count);
   who.reserve(count);
   for (int s = 0; s < smax; ++s) {
     for (int mode : {IUWT_SMOOTH, IUWT_DECOMP}) {
       calls.clear();
       who.clear();
+      pairs.clear();
+      pwho.clear();
+      Plan pl;
+      const bool pair_plan = fuse == 1 && mode == IUWT_DECOMP && dec2_plan(h, w, s, &pl);
       for (int k = 0; k < count; ++k) {
         const IuwtDecompJob& j = jobs[k];
         float* ping = j.work + (size_t)state[k].slot * plane;
@@ -977,8 +1342,21 @@ static void decompose_batch_impl(IuwtDecompJob* jobs, int count, int h, int w, h
           if (s >= j.scale_adjust) continue;
           c.out_c = ping;
         } else {
-          if (s < j.scale_adjust || s >= j.scale_count) continue;
+          if (s < j.scale_adjust || s >= j.scale_count || s < state[k].done) continue;
           const int kk = s - j.scale_adjust;
+          if (pair_plan && s + 1 < j.scale_count && kk % 2 == 0 && j.detail && !j.ref) {
+            Dec2Job pj{};
+            pj.in = state[k].cur;
+            pj.out_c = (s + 1 == j.scale_count - 1) ? j.smoothed : ping;
+            pj.out_w0 = j.detail + (size_t)kk * plane;
+            pj.out_w1 = j.detail + (size_t)(kk + 1) * plane;
+            pj.mask0 = j.mask ? j.mask + (size_t)kk * plane : nullptr;
+            pj.mask1 = j.mask ? j.mask + (size_t)(kk + 1) * plane : nullptr;
+            pj.pred = j.pred;
+            pairs.push_back(pj);
+            pwho.push_back(k);
+            continue;
+          }
           const bool last = (s == j.scale_count - 1);
           // the direct-stencil form of an over-long scale needs C even when it is not kept
           c.out_c = (last && (j.smoothed || !iuwt_over_long(h, w, s))) ? j.smoothed : ping;
@@ -989,6 +1367,17 @@ static void decompose_batch_impl(IuwtDecompJob* jobs, int count, int h, int w, h
         }
         calls.push_back(c);
         who.push_back(k);
+      }
+      if (!pairs.empty()) {
+        dec2_batch(pairs.data(), (int)pairs.size(), h, w, s, pl, st);
+        for (int k : pwho) {
+          State& sk = state[k];
+          sk.done = s + 2;
+          if (s + 1 != jobs[k].scale_count - 1) {
+            sk.cur = jobs[k].work + (size_t)sk.slot * plane;
+            sk.slot ^= 1;
+          }
+        }
       }
       if (calls.empty()) continue;
       const int nb = iuwt_scale_batch(calls.data(), (int)calls.size(), st);
@@ -1288,6 +1677,9 @@ std::vector<TuneParams> walk_candidates(int h, int w, bool recomp) {
   const int pair_default = env_int("PMS_IUWT_PAIR", 2);
   std::vector<int> blocks = fixed_blocks ? std::vector<int>{tuning().target_blocks} : std::vector<int>{1024, 2048, 4096};
   std::vector<int> pairs = (recomp && !fixed_pair) ? std::vector<int>{2, 0} : std::vector<int>{pair_default};
+  // for a decomposition the field selects two-scale launches (1) or one scale per launch (0); it is
+  // not timed (the two-scale walk measured slower at every tested shape), PMS_IUWT_DEC2 sets it
+  if (!recomp) pairs = std::vector<int>{env_int("PMS_IUWT_DEC2", kDec2Default)};
   for (int p : pairs)
     for (int b : blocks) c.push_back(TuneParams{b, p});
   return c;

@@ -129,6 +129,46 @@ def test_moresane_two_scale_recomposition_is_exact(hip, monkeypatch):
     assert out[0].trace == out[1].trace
 
 
+@pytest.mark.parametrize("shape,scales,adjust", [
+    ((1024, 1024), 6, 0), ((300, 260), 5, 0), ((257, 333), 4, 1), ((768, 1040), 5, 1), ((160, 420), 5, 0),
+    ((1040, 600), 6, 2), ((2048, 2048), 7, 0),
+])
+def test_iuwt_two_scale_decomposition_is_exact(hip, shape, scales, adjust, monkeypatch):
+    """Pairs of detail scales decomposed in one launch (the intermediate smoothed plane only in LDS)
+    give the same bits as one launch per scale, borders included."""
+    say("iuwt decomposition pairs vs single scales %s S=%d adj=%d" % (shape, scales, adjust))
+    x = np.random.default_rng(9).standard_normal(shape).astype(np.float32)
+    monkeypatch.setenv("PMS_AUTOTUNE", "0")
+    monkeypatch.setenv("PMS_IUWT_DEC2", "1")
+    det_f, sm_f = iuwt.iuwt_decomposition(x, scales, adjust, store_smoothed=True)
+    monkeypatch.setenv("PMS_IUWT_DEC2", "0")
+    det_s, sm_s = iuwt.iuwt_decomposition(x, scales, adjust, store_smoothed=True)
+    np.testing.assert_array_equal(det_f, det_s)
+    np.testing.assert_array_equal(sm_f, sm_s)
+    rdet, rsm = oiuwt.iuwt_decomposition(x, scales, adjust, store_smoothed=True)
+    for k in range(det_f.shape[0]):
+        assert rel(det_f[k], rdet[k]) < 1e-5, (k, rel(det_f[k], rdet[k]))
+    assert rel(sm_f, rsm) < 1e-6
+
+
+def test_moresane_two_scale_decomposition_is_exact(hip, monkeypatch):
+    """Masked decompositions inside the minor loop: the whole deconvolution is bit-identical with and
+    without two-scale decomposition launches."""
+    say("moresane 1024^2, S=4, two-scale decomposition on/off")
+    from pymoresane_amd.main import DataImage
+    dirty, psf = synth.make_pair(1024, 32, n_points=30, n_blobs=3)
+    kw = dict(scale_count=4, loop_gain=0.2, major_loop_miter=2, minor_loop_miter=12, enforce_positivity=True)
+    out = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("PMS_IUWT_DEC2", flag)
+        img = DataImage(dirty, psf, None, 1.0, 1.0)
+        img.moresane(**kw)
+        out.append(img)
+    np.testing.assert_array_equal(out[0].model, out[1].model)
+    np.testing.assert_array_equal(out[0].residual, out[1].residual)
+    assert out[0].trace == out[1].trace
+
+
 def test_iuwt_round_trip_at_full_size(hip):
     """4096^2, S=7, store_smoothed: decomposition then recomposition reproduces the input."""
     say("iuwt round trip 4096^2 S=7")
