@@ -123,7 +123,7 @@ def traffic(args):
     kib_f, kib_w = sum(cf) / len(cf), sum(cw) / len(cw)
     f_scale = args.copy_bytes / (kib_f * 1024.0)
     w_scale = args.copy_bytes / (kib_w * 1024.0)
-    fetch_name = "read bytes from the request counts" if args.fetch_requests else "FETCH_SIZE"
+    fetch_name = "(from its read requests)" if args.fetch_requests else "FETCH_SIZE ="
 
     def table(rows):
         t = collections.defaultdict(list)
@@ -135,7 +135,7 @@ def traffic(args):
     out = ["# HBM traffic per launch: %s" % args.title, "",
            "Commands (repo root, on an MI355X), one counter per pass:", "",
            "```", open(args.commands).read().rstrip(), "```", "",
-           "Calibration: the plain copy kernel (k_copy) moving %d bytes each way reads %s = %.1f KiB and "
+           "Calibration: the plain copy kernel (k_copy) moving %d bytes each way reads %s %.1f KiB and "
            "WRITE_SIZE = %.1f KiB per dispatch, so the scale factors to true bytes are **fetch x%.3f** and "
            "**write x%.3f**." % (args.copy_bytes, fetch_name, kib_f, kib_w, f_scale, w_scale), "",
            "| kernel | grid | dispatches | %s KiB (raw avg) | WRITE KiB (raw avg) | traffic MB (calibrated) |"

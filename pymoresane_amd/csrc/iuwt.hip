@@ -1312,8 +1312,10 @@ static void decompose_batch_impl(IuwtDecompJob* jobs, int count, int h, int w, h
     smax = std::max(smax, j.scale_count);
   }
   // detail scales in pairs from the finest up where a two-scale launch applies (1), or one scale per
-  // launch (0): the autotuner's choice, else PMS_IUWT_DEC2 (the planes are identical either way)
-  const int fuse = t_active ? t_active->pair : env_int("PMS_IUWT_DEC2", kDec2Default);
+  // launch (0): PMS_IUWT_DEC2 when set, else the autotuner's record, else the default (the planes are
+  // identical either way)
+  const int fuse = getenv("PMS_IUWT_DEC2") ? env_int("PMS_IUWT_DEC2", kDec2Default)
+                                           : (t_active ? t_active->pair : kDec2Default);
   std::vector<IuwtScaleCall> calls;
   std::vector<int> who;
   std::vector<Dec2Job> pairs;
@@ -1429,8 +1431,9 @@ static void recompose_batch_impl(const IuwtRecompJob* jobs, int count, int h, in
   std::vector<State> state(count);
   size_t rounds = 0;
   // PMS_IUWT_PAIR=0 keeps every scale in its own launch, 1 pairs detail scales from the coarsest
-  // down, 2 from the finest up (read per call; the results are identical)
-  const int pair = t_active ? t_active->pair : env_int("PMS_IUWT_PAIR", 2);
+  // down, 2 from the finest up (read per call, and it overrides the autotuner's record; the results
+  // are identical)
+  const int pair = getenv("PMS_IUWT_PAIR") ? env_int("PMS_IUWT_PAIR", 2) : (t_active ? t_active->pair : 2);
   for (int k = 0; k < count; ++k) {
     const IuwtRecompJob& j = jobs[k];
     PMS_REQUIRE(j.nplanes >= 1 && j.scale_adjust >= 0, "recomposition needs at least one detail plane");
