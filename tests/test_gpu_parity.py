@@ -506,6 +506,25 @@ def test_moresane_small_linear(hip):
     assert rel(img.residual, ref.residual) < 1e-4
 
 
+@pytest.mark.parametrize("case", ["noise_only", "all_zero"])
+def test_moresane_without_sources(hip, case):
+    """Inputs with nothing to deconvolve: noise alone (no significant coefficient at the first major
+    iteration, then no work: the oracle's trace is nocoeff, nowork) and an all-zero image.  The trace,
+    model and residual are the oracle's."""
+    say("moresane 64^2 without sources: %s" % case)
+    _, psf = synth.make_pair(64, 23, n_points=10, n_blobs=2)
+    if case == "noise_only":
+        dirty = np.random.default_rng(3).normal(0, 1e-3, (64, 64)).astype(np.float32)
+    else:
+        dirty = np.zeros((64, 64), np.float32)
+    img, ref = run_pair(dirty, psf, scale_count=4, enforce_positivity=True)
+    assert discrete(img.trace) == discrete(ref.trace)
+    if case == "noise_only":
+        assert [e[0] for e in ref.trace] == ["nocoeff", "nowork"]
+    np.testing.assert_array_equal(img.model, ref.model.astype(np.float32))
+    np.testing.assert_array_equal(img.residual, ref.residual.astype(np.float32))
+
+
 def test_moresane_small_circular_options(hip):
     say("moresane 64^2 circular + options")
     dirty, psf = synth.make_pair(64, 22, psf_factor=1, n_points=8, n_blobs=1)
