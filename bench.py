@@ -408,6 +408,8 @@ def main():
     ap.add_argument("--c0-only", action="store_true", help="run only the c0 leg (with its CPU baseline unless "
                                                             "--no-cpu), print its line, exit")
     ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate of the served form, per s")
+    ap.add_argument("--served-only", action="store_true", help="run only the served form (c4) and print its object")
     ap.add_argument("--max-batch", type=int, default=32, help="jobs per lockstep batch (batched/served forms)")
     ap.add_argument("--calibrate-runs", type=int, nargs="?", const=64, metavar="BYTES",
                     help="only run a copy that reads runs of BYTES (default 64) bytes, every other run of a "
@@ -473,6 +475,9 @@ def main():
         return
     if args.c3_class:
         print(json.dumps(bench_c3_class(args.c3_class, args.max_batch, args.c3_plain)))
+        return
+    if args.served_only:
+        print(json.dumps(bench_served(args.serve_requests, args.max_batch, rate=args.serve_rate)))
         return
     if args.c0_only:
         print(json.dumps(bench_c0(cpu=not args.no_cpu)))
@@ -636,7 +641,7 @@ def main():
         buf.free()
     if not args.no_batch:
         res["batched"] = bench_batched(args.max_batch)
-        res["served"] = bench_served(args.serve_requests, args.max_batch)
+        res["served"] = bench_served(args.serve_requests, args.max_batch, rate=args.serve_rate)
     if not args.no_c0:
         log("c0 plain deconvolution%s" % ("" if args.no_cpu else " and its float64 oracle run (1 core)"))
         res["c0"] = bench_c0(cpu=not args.no_cpu)
