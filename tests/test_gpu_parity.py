@@ -336,6 +336,20 @@ def test_fft_convolve_largest_linear_frame(hip):
     assert rel(conv.fft_convolve(a + b, spec), conv.fft_convolve(a, spec) + conv.fft_convolve(b, spec)) < 1e-6
 
 
+@pytest.mark.parametrize("n,mode", [(4096, "linear"), (8192, "circular")])
+def test_fft_convolve_large_frame_matches_oracle(hip, n, mode):
+    """Random image and PSF at the 8192^2 frame, linear (the 4096^2 microbenchmark) and circular: the
+    frames whose row passes take two row groups per workgroup, against the float64 oracle."""
+    say("fft_convolve n=%d %s vs oracle" % (n, mode))
+    rng = np.random.default_rng(14)
+    x = rng.standard_normal((n, n), dtype=np.float32)
+    p = 2 * n if mode == "linear" else n
+    psf = rng.standard_normal((p, p), dtype=np.float32)
+    got = conv.fft_convolve(x, conv.psf_spectrum(psf), conv_mode=mode)
+    ref = oconv.fft_convolve(x, oconv.psf_spectrum(psf), mode)
+    assert rel(got, ref) < 2e-6, rel(got, ref)
+
+
 @pytest.mark.parametrize("n", [64, 8192])
 def test_fft_convolve_circular_delta_response(hip, n):
     """Circular mode (P = n) answers a delta at (a, b) with the PSF rolled by (a + n/2, b + n/2) (the
