@@ -678,7 +678,10 @@ __global__ __launch_bounds__(ColCfg<PG>::T) void k_conv_cols(ColBatch bt, int n_
     for (int e = 0; e < (LIN ? P / 2 : P); ++e) out[e] = x[e];
   } else {
     Stage0<PG, NT, LIN> s0;
-    s0.load(lt, tw, [&](int e) { return live ? col[e] : make_float2(0.f, 0.f); });
+    // unconditional loads (a dead block reads column 0 and stores nothing): loads under a branch
+    // leave the compiler's wait counts conservative, so that stage 0 would also wait for the PSF
+    // segment requested after its inputs
+    s0.load(lt, tw, [&](int e) { return col[e]; });
     StageTw<PG, NT> stw;
     stw.load(lt, tw);
     constexpr int PER = StageTw<PG, NT>::PER;
@@ -687,7 +690,7 @@ __global__ __launch_bounds__(ColCfg<PG>::T) void k_conv_cols(ColBatch bt, int n_
     for (int j = 0; j < PER; ++j) {
       const int b = lt + j * NT;
 #pragma unroll
-      for (int m = 0; m < 16; ++m) sv[j][m] = (live && b < Pl::NBF) ? sc[16 * b + m] : make_float2(0.f, 0.f);
+      for (int m = 0; m < 16; ++m) sv[j][m] = (b < Pl::NBF) ? sc[16 * b + m] : make_float2(0.f, 0.f);
     }
     s0.forward(sm, lt);
     __syncthreads();
