@@ -128,3 +128,22 @@ def test_server_rejects_oversized_request(hip):
     with pytest.raises(Exception, match="budget"):
         srv.submit(d, p, **KW)
     srv.close()
+
+
+def test_batch_of_4096_frames_equals_alone(hip):
+    """Two 4096^2 jobs with 8192^2 PSFs (linear mode) in one batch: the batched launches at the 8192
+    frame, the largest of the served mix, give every job the bits it gets alone."""
+    say("batch vs alone, 8192-frame convolutions, two jobs per batch")
+    kw = dict(loop_gain=0.2, major_loop_miter=2, enforce_positivity=True, scale_count=3)
+    ps = [synth.make_pair(4096, 700 + i, n_points=8, n_blobs=1) for i in range(2)]
+    ref = []
+    for d, p in ps:
+        img = DataImage(d, p, None, 1.0, 1.0)
+        img.moresane(**kw)
+        ref.append((img.model, img.residual))
+    jobs = batch.DeviceJobs(ps, [kw] * len(ps))
+    stats = batch.run_batch(jobs, n_workers=1, max_batch=2)
+    assert all(s[0] == 0 for s in stats)
+    for (m, r), (m0, r0) in zip(jobs.results(), ref):
+        np.testing.assert_array_equal(m, m0)
+        np.testing.assert_array_equal(r, r0)
