@@ -575,42 +575,39 @@ __global__ __launch_bounds__(kT) void k_ccl_compress(int* __restrict__ lab, size
     if (l[i] >= 0) l[i] = ccl_find(l, (int)i);
 }
 
-// mark roots of objects holding a seed pixel (value >= tol*max, inside the coarser kept mask)
+// One step of the per-scale keep rule, coarsest scale first, in one pass over the pixels: applies
+// scale sa's marks (mask = pixel's object kept, extracted = mask * w) and marks the roots of scale
+// sm = sa - 1's objects that hold a seed pixel (value >= tol * max) inside sa's kept mask.  sa < 0:
+// mark only (the coarsest scale, which has no coarser mask); sm < 0: apply only.  A mark is the
+// scale's number plus one, in one of two flag planes by the scale's parity, so a launch reads the
+// marks of one plane while it writes the other's and no plane needs clearing between scales.
 template <class T>
-__global__ __launch_bounds__(kT) void k_keep_mark(const T* __restrict__ w, const int* __restrict__ lab,
-                                                  const uint8_t* __restrict__ coarse_mask, const T* __restrict__ smax,
-                                                  int s, double tol, int neg_comp, size_t plane,
-                                                  uint8_t* __restrict__ keep, const int* pred) {
+__global__ __launch_bounds__(kT) void k_keep_step(const T* __restrict__ w, const int* __restrict__ lab, int sa, int sm,
+                                                  const T* __restrict__ smax, double tol, int neg_comp, size_t plane,
+                                                  uint8_t* __restrict__ keep2, size_t kstride,
+                                                  uint8_t* __restrict__ mask, T* __restrict__ extracted,
+                                                  const int* pred) {
   if (pred_off(pred)) return;
-  const double lim = tol * (double)smax[s];
-  const T* x = w + (size_t)s * plane;
-  const int* l = lab + (size_t)s * plane;
+  const double lim = sm >= 0 ? tol * (double)smax[sm] : 0.0;
+  const uint8_t* ka = keep2 + (size_t)(sa & 1) * kstride;
+  uint8_t* km = keep2 + (size_t)(sm & 1) * kstride;
   for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < plane; i += (size_t)gridDim.x * kT) {
-    const int root = l[i];
-    if (root < 0) continue;
-    const T v = neg_comp ? (T)fabs(x[i]) : x[i];
-    if (!((double)v >= lim)) continue;
-    if (coarse_mask && coarse_mask[i] == 0) continue;
-    keep[root] = 1;
-  }
-}
-
-// mask = object kept; extracted = mask * w; keep flags are consumed and cleared
-template <class T>
-__global__ __launch_bounds__(kT) void k_keep_apply(const T* __restrict__ w, const int* __restrict__ lab, int s,
-                                                   size_t plane, const uint8_t* __restrict__ keep,
-                                                   uint8_t* __restrict__ mask, T* __restrict__ extracted,
-                                                   const int* pred) {
-  if (pred_off(pred)) return;
-  const T* x = w + (size_t)s * plane;
-  const int* l = lab + (size_t)s * plane;
-  uint8_t* m = mask + (size_t)s * plane;
-  T* e = extracted ? extracted + (size_t)s * plane : nullptr;
-  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < plane; i += (size_t)gridDim.x * kT) {
-    const int root = l[i];
-    const uint8_t k = (root >= 0) ? keep[root] : (uint8_t)0;
-    m[i] = k;
-    if (e) e[i] = k ? x[i] : (T)0;
+    bool coarse_kept = true;
+    if (sa >= 0) {
+      const size_t o = (size_t)sa * plane + i;
+      const int root = lab[o];
+      const bool k = root >= 0 && ka[root] == (uint8_t)(sa + 1);
+      mask[o] = k;
+      if (extracted) extracted[o] = k ? w[o] : (T)0;
+      coarse_kept = k;
+    }
+    if (sm >= 0 && coarse_kept) {
+      const size_t o = (size_t)sm * plane + i;
+      const int root = lab[o];
+      if (root < 0) continue;
+      const T v = neg_comp ? (T)fabs(w[o]) : w[o];
+      if ((double)v >= lim) km[root] = (uint8_t)(sm + 1);
+    }
   }
 }
 
@@ -1022,12 +1019,12 @@ void max_final(const double* pmax, int nb, int S, double* out_max, const double*
   max_final_t(pmax, nb, S, out_max, energy, nsm, pred, st);
 }
 
-// workspace layout: [keep flags: one plane of bytes][maxima: 64 values][partial maxima: 16 x 2048 values]
-// [labels: S planes of int]; everything before the labels sits at fixed offsets (values sized for
-// double, so one layout serves both element types).
+// workspace layout: [keep flags: two planes of bytes][maxima: 64 values][partial maxima: 16 x 2048
+// values][labels: S planes of int]; everything before the labels sits at fixed offsets (values sized
+// for double, so one layout serves both element types).
 static size_t keep_bytes(size_t plane) { return ((plane + 255) / 256) * 256; }
 static size_t ext_labels_offset(size_t plane) {
-  return keep_bytes(plane) + 64 * sizeof(double) + (size_t)kMaxPlanes * 2048 * sizeof(double);
+  return 2 * keep_bytes(plane) + 64 * sizeof(double) + (size_t)kMaxPlanes * 2048 * sizeof(double);
 }
 
 size_t extract_work_bytes(int S, int h, int w) {
@@ -1037,7 +1034,8 @@ size_t extract_work_bytes(int S, int h, int w) {
 
 void extract_work_init(void* work, int, int h, int w, hipStream_t st) {
   const size_t plane = (size_t)h * w;
-  hipLaunchKernelGGL(k_clear_u8, dim3(ew_grid(plane)), dim3(kT), 0, st, (uint8_t*)work, plane, (const int*)nullptr);
+  hipLaunchKernelGGL(k_clear_u8, dim3(ew_grid(2 * plane)), dim3(kT), 0, st, (uint8_t*)work, 2 * keep_bytes(plane),
+                     (const int*)nullptr);
   PMS_LAUNCH_CHECK();
 }
 
@@ -1048,12 +1046,13 @@ static void source_extract_t(const T* w, int S, int h, int wd, double tol, int n
   PMS_REQUIRE(S >= 1 && S <= kMaxPlanes, "source extraction handles 1..16 planes per call");
   const size_t plane = (size_t)h * wd;
   uint8_t* keep = (uint8_t*)work;
-  T* smax = (T*)((char*)work + keep_bytes(plane));
-  T* pmax = (T*)((char*)work + keep_bytes(plane) + 64 * sizeof(double));
+  const size_t kstride = keep_bytes(plane);
+  T* smax = (T*)((char*)work + 2 * kstride);
+  T* pmax = (T*)((char*)work + 2 * kstride + 64 * sizeof(double));
   int* lab = (int*)((char*)work + ext_labels_offset(plane));
   const int gx = threshold_grid(plane);
   // keep flags start cleared whatever the workspace held before
-  hipLaunchKernelGGL(k_clear_u8, dim3(ew_grid(plane)), dim3(kT), 0, st, keep, plane, pred);
+  hipLaunchKernelGGL(k_clear_u8, dim3(ew_grid(2 * plane)), dim3(kT), 0, st, keep, 2 * kstride, pred);
   PMS_LAUNCH_CHECK();
   plane_max_t(w, S, plane, neg_comp, pmax, pred, st);
   max_final_t(pmax, gx, S, smax, (const double*)nullptr, (double*)nullptr, pred, st);
@@ -1065,15 +1064,10 @@ static void source_extract_t(const T* w, int S, int h, int wd, double tol, int n
   PMS_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_ccl_compress, dim3(gl, S), dim3(kT), 0, st, lab, plane, pred);
   PMS_LAUNCH_CHECK();
-  for (int s = S - 1; s >= 0; --s) {
-    const uint8_t* coarse = (s == S - 1) ? nullptr : mask + (size_t)(s + 1) * plane;
-    hipLaunchKernelGGL(k_keep_mark<T>, dim3(gl), dim3(kT), 0, st, w, (const int*)lab, coarse, (const T*)smax, s, tol,
-                       neg_comp, plane, keep, pred);
-    PMS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_keep_apply<T>, dim3(gl), dim3(kT), 0, st, w, (const int*)lab, s, plane,
-                       (const uint8_t*)keep, mask, extracted, pred);
-    PMS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_clear_u8, dim3(ew_grid(plane)), dim3(kT), 0, st, keep, plane, pred);
+  // keep rule from the coarsest scale down: mark S-1; apply s and mark s-1; ...; apply 0
+  for (int sa = S; sa >= 0; --sa) {
+    hipLaunchKernelGGL(k_keep_step<T>, dim3(gl), dim3(kT), 0, st, w, (const int*)lab, sa < S ? sa : -1, sa - 1,
+                       (const T*)smax, tol, neg_comp, plane, keep, kstride, mask, extracted, pred);
     PMS_LAUNCH_CHECK();
   }
   if (labels_out) {
