@@ -880,7 +880,7 @@ struct IuwtTuning {
   IuwtTuning()
       : qmax(env_int("PMS_IUWT_QMAX", 32)),
         krows(env_int("PMS_IUWT_KROWS", 0)),
-        target_blocks(env_int("PMS_IUWT_BLOCKS", 2048)),
+        target_blocks(env_int("PMS_IUWT_BLOCKS", 512)),
         small_pixels(env_int("PMS_IUWT_SMALL", 256 * 256)),
         pair_min_rows(env_int("PMS_IUWT_PAIR_MIN_ROWS", 8)) {}
 };
@@ -903,6 +903,15 @@ constexpr int kDec2Default = 0;
 thread_local const TuneParams* t_active = nullptr;
 
 int plan_target_blocks() { return t_active ? t_active->target_blocks : tuning().target_blocks; }
+
+// Output rows per walk once the walk count is fixed: the same number of segments, of equal length
+// (a length cap can leave a short last segment, e.g. 56 + 8 rows, whose walk pays the same lead-in
+// while the long one sets the launch's time), unless that would go below kmin.
+int balanced_rows(int mr, int k, int kmin) {
+  if (k <= 0 || getenv("PMS_IUWT_KROWS")) return k;
+  const int kb = ceil_div(mr, ceil_div(mr, k));
+  return kb >= kmin ? kb : k;
+}
 
 Plan make_plan(int h, int w, int scale, int halo) {
   Plan pl;
@@ -931,6 +940,7 @@ Plan make_plan(int h, int w, int scale, int halo) {
   }
   if (k > kWalkMaxSteps - 2 * halo) k = kWalkMaxSteps - 2 * halo;  // a walk's steps fit one wave's lanes
   if (k > mr) k = mr;
+  k = balanced_rows(mr, k, 4 * halo);
   pl.k_rows = k;
   pl.n_cg = ceil_div(d, q);
   pl.n_tb = ceil_div(teeth, tj);
@@ -1013,6 +1023,7 @@ bool recomp2_plan(int h, int w, int s, Plan* out) {
   }
   if (k > kWalkMaxSteps - 12) k = kWalkMaxSteps - 12;  // a walk's steps fit one wave's lanes
   if (k > mr) k = mr;
+  k = balanced_rows(mr, k, 24);
   pl.k_rows = k;
   pl.n_cg = ceil_div(d, q);
   pl.n_tb = ceil_div(teeth, tj);
@@ -1106,6 +1117,7 @@ bool dec2_plan(int h, int w, int s, Plan* out) {
   }
   if (k > kDec2MaxSteps - 21) k = kDec2MaxSteps - 21;  // a walk's steps fit two table registers
   if (k > mr) k = mr;
+  k = balanced_rows(mr, k, 40);
   pl.k_rows = k;
   pl.n_cg = ceil_div(d, q);
   pl.n_tb = ceil_div(teeth, tj);
@@ -1678,7 +1690,8 @@ std::vector<TuneParams> walk_candidates(int h, int w, bool recomp) {
   const bool fixed_blocks = getenv("PMS_IUWT_BLOCKS") != nullptr;
   const bool fixed_pair = getenv("PMS_IUWT_PAIR") != nullptr;
   const int pair_default = env_int("PMS_IUWT_PAIR", 2);
-  std::vector<int> blocks = fixed_blocks ? std::vector<int>{tuning().target_blocks} : std::vector<int>{1024, 2048, 4096};
+  std::vector<int> blocks =
+      fixed_blocks ? std::vector<int>{tuning().target_blocks} : std::vector<int>{512, 1024, 2048, 4096};
   std::vector<int> pairs = (recomp && !fixed_pair) ? std::vector<int>{2, 0} : std::vector<int>{pair_default};
   // for a decomposition the field selects two-scale launches (1) or one scale per launch (0); it is
   // not timed (the two-scale walk measured slower at every tested shape), PMS_IUWT_DEC2 sets it
