@@ -953,17 +953,22 @@ Plan make_plan(int h, int w, int scale, int halo) {
 }
 
 constexpr int kWalkGroup = 2;
+// rows per prefetch group of a one-scale decomposition walk: two groups of four rows in flight hide
+// more of the first scale's HBM latency (its input is the only plane not just written to the caches);
+// the recompositions keep groups of two, which measured faster there
+constexpr int kDecompGroup = 4;
 
 template <int MODE, int FL>
 void launch_walk_fl(ScaleParams& sp, int count, const Plan& pl, int halo, hipStream_t st) {
+  constexpr int K = MODE == IUWT_DECOMP ? kDecompGroup : kWalkGroup;
   const int elems = (pl.tj + 2 * halo) << pl.q_log2;
   dim3 grid(pl.blocks, count), block(kThreads);
   if (elems <= 2 * kThreads)
-    hipLaunchKernelGGL((k_iuwt_walk<MODE, FL, kWalkGroup, 2>), grid, block, pl.lds_bytes, st, sp);
+    hipLaunchKernelGGL((k_iuwt_walk<MODE, FL, K, 2>), grid, block, pl.lds_bytes, st, sp);
   else if (elems <= 4 * kThreads)
-    hipLaunchKernelGGL((k_iuwt_walk<MODE, FL, kWalkGroup, 4>), grid, block, pl.lds_bytes, st, sp);
+    hipLaunchKernelGGL((k_iuwt_walk<MODE, FL, K, 4>), grid, block, pl.lds_bytes, st, sp);
   else
-    hipLaunchKernelGGL((k_iuwt_walk<MODE, FL, kWalkGroup, 8>), grid, block, pl.lds_bytes, st, sp);
+    hipLaunchKernelGGL((k_iuwt_walk<MODE, FL, K, 8>), grid, block, pl.lds_bytes, st, sp);
   PMS_LAUNCH_CHECK();
 }
 
