@@ -209,10 +209,9 @@ __global__ __launch_bounds__(kThreads) void k_iuwt_walk(ScaleParams sp) {
   const ScaleJob a = sp.job[blockIdx.y];
   if (a.pred && *a.pred == 0) return;
 
-  // two row buffers of ML*256 slots (every lane writes its slots)
+  // 2K row buffers of ML*256 slots (every lane writes its slots): a group's K input rows are staged
+  // together behind one barrier, consecutive groups in alternate halves
   extern __shared__ float lds[];
-  float* buf0 = lds;
-  float* buf1 = lds + ML * kThreads;
   const int Q = 1 << sp.q_log2;
   const int TJ = sp.tj;
   const int row_elems = (TJ + 2 * H) * Q;
@@ -300,11 +299,19 @@ __global__ __launch_bounds__(kThreads) void k_iuwt_walk(ScaleParams sp) {
   for (int k = 0; k < 5; ++k) wx[k] = 0.f;
   double acc = 0.0;
 
-  auto step = [&](const Group& G, int u, int s) {
-    float* cur = (u & 1) ? buf1 : buf0;
+  // a wave writes group g's half only after the barrier of group g - 1, which every wave reaches only
+  // once it has read group g - 2 (the same half)
+  auto stage = [&](const Group& G, int g) {
+    float* half = lds + (size_t)((g & 1) * K) * ML * kThreads;
 #pragma unroll
-    for (int m = 0; m < ML; ++m) cur[t + m * kThreads] = G.row[u][m];
+    for (int u = 0; u < K; ++u) {
+#pragma unroll
+      for (int m = 0; m < ML; ++m) half[(u * ML + m) * kThreads + t] = G.row[u][m];
+    }
     __syncthreads();
+  };
+  auto step = [&](const Group& G, int u, int s) {
+    const float* cur = lds + (size_t)(((s / K) & 1) * K + u) * ML * kThreads;
     const float* rowp = cur + tap_lds;
     float ha = 0.f, hb = 0.f, hx = 0.f;
 #pragma unroll
@@ -360,10 +367,12 @@ __global__ __launch_bounds__(kThreads) void k_iuwt_walk(ScaleParams sp) {
   load_group(GA, 0);
   for (int g = 0; g < n_groups; g += 2) {
     load_group(GB, g + 1);
+    stage(GA, g);
 #pragma unroll
     for (int u = 0; u < K; ++u) step(GA, u, g * K + u);
     load_group(GA, g + 2);
     if (g + 1 >= n_groups) break;
+    stage(GB, g + 1);
 #pragma unroll
     for (int u = 0; u < K; ++u) step(GB, u, (g + 1) * K + u);
   }
@@ -963,12 +972,15 @@ void launch_walk_fl(ScaleParams& sp, int count, const Plan& pl, int halo, hipStr
   constexpr int K = MODE == IUWT_DECOMP ? kDecompGroup : kWalkGroup;
   const int elems = (pl.tj + 2 * halo) << pl.q_log2;
   dim3 grid(pl.blocks, count), block(kThreads);
+  // 2K staged rows of ML slots per lane
+  const size_t lds = [&](int ml) { return (size_t)2 * K * ml * kThreads * sizeof(float); }(
+      elems <= 2 * kThreads ? 2 : (elems <= 4 * kThreads ? 4 : 8));
   if (elems <= 2 * kThreads)
-    hipLaunchKernelGGL((k_iuwt_walk<MODE, FL, K, 2>), grid, block, pl.lds_bytes, st, sp);
+    hipLaunchKernelGGL((k_iuwt_walk<MODE, FL, K, 2>), grid, block, lds, st, sp);
   else if (elems <= 4 * kThreads)
-    hipLaunchKernelGGL((k_iuwt_walk<MODE, FL, K, 4>), grid, block, pl.lds_bytes, st, sp);
+    hipLaunchKernelGGL((k_iuwt_walk<MODE, FL, K, 4>), grid, block, lds, st, sp);
   else
-    hipLaunchKernelGGL((k_iuwt_walk<MODE, FL, K, 8>), grid, block, pl.lds_bytes, st, sp);
+    hipLaunchKernelGGL((k_iuwt_walk<MODE, FL, K, 8>), grid, block, lds, st, sp);
   PMS_LAUNCH_CHECK();
 }
 
