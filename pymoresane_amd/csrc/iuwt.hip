@@ -192,8 +192,10 @@ constexpr int kWalkMaxSteps = 64;
 // Row walk with group pipelining over the comb/phase decomposition described at the top of this
 // file; the walk is cut into groups of K rows: while a group's K steps run, the input
 // rows and per-output operands (mask, reference, detail plane) of the next group are already in
-// flight, so the loop body never drains the memory pipe.  Loads never branch (dead slots read a
-// valid column) and stores never branch (no-output steps write to g_walk_sink).  Row offsets of
+// flight, so the loop body never drains the memory pipe.  A group's K input rows are staged in LDS
+// together, so the workgroup meets at one barrier per group rather than one per row.  Loads never
+// branch (dead slots read a valid column) and stores never branch (no-output steps write to
+// g_walk_sink).  Row offsets of
 // every step come from per-lane tables (readlane) and loads are buffer loads with the row offset
 // in a scalar register, which keeps per-step scalar and address arithmetic to a few instructions.
 template <int MODE, int FL, int K, int ML>
