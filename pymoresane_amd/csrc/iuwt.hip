@@ -29,7 +29,8 @@
 // shape allows (k_iuwt_recomp2: the coarser scale is the same filter at
 // spacing 2 in the finer scale's frame), with results bit-identical to one
 // launch per scale.  Planes of at most 256^2 pixels use a direct-stencil
-// kernel instead of the walks.  An autotuner (end of this file) picks the
+// kernel instead of the walks, and pairs of scales there run as one launch
+// (k_iuwt_small2_dec / k_iuwt_small2_rec, also bit-identical).  An autotuner (end of this file) picks the
 // walks' workgroup count and the recomposition pairing per call signature.
 #include <algorithm>
 #include <cmath>
@@ -1198,6 +1199,294 @@ void dec2_batch(const Dec2Job* jobs, int count, int h, int w, int s, const Plan&
   }
 }
 
+// ---- two scales per launch on small planes ---------------------------------------------------
+// Planes in the direct-stencil range take a pair of scales (s, s+1) in one launch.  A workgroup owns
+// a tile of one phase class: rows pr + d*i and columns pc + d*j for 16 consecutive lattice indices i
+// and j (d = 2^s).  On that lattice scale s is the undilated filter and scale s+1 the same filter at
+// spacing 2, so the tile needs its input only within 10 lattice steps (decomposition) or 4
+// (recomposition); that region is staged in LDS once, and the intermediate plane -- C_{s+1} of a
+// decomposition, T = h_{s+1} R + w_{s+1} of a recomposition -- is computed from it in LDS for the
+// tile and a halo of 8 (2) lattice steps and never reaches memory.  Region and halo positions keep
+// their unfolded lattice coordinates: a staged input value is the symmetric extension there, and an
+// intermediate value at a mirrored position (an odd number of reflections away) accumulates its
+// taps in reverse order, which makes it the same sum, in the same order, as the one-scale kernel
+// forms at the folded position.  Every plane is therefore bit-identical to two k_iuwt_small launches.
+constexpr int kTile2 = 16;  // tile side in lattice steps (one output pixel per thread)
+
+template <class J>
+struct Small2Params {
+  J job[kMaxBatch];  // Dec2Job or PairJob
+  int h, w, d;
+  int ntr, ntc;      // tiles per phase along rows and columns
+};
+
+// Folded plane index of lattice coordinates origin + step*(first + k), k in [0, n), for k = lane;
+// the index is stored with its mirror flag as ~index when the position lies on a mirrored period.
+// Any integer coordinate maps (the extension is periodic with period 2*size).
+__device__ __forceinline__ void ext_indices(int* out, int n, int origin, int step, int first, int size) {
+  const int k = threadIdx.x & 63;
+  if (k < n) {
+    const int p = 2 * size;
+    int m = (origin + step * (first + k)) % p;
+    if (m < 0) m += p;
+    out[k] = m >= size ? ~(p - 1 - m) : m;
+  }
+}
+__device__ __forceinline__ int ext_pos(int e) { return e < 0 ? ~e : e; }
+
+// Stage the RS x RS region of lattice positions (ry[i], rx[j]) of plane `in` into LDS rows of RP
+// floats.  Every thread issues the same number of loads (the last ones clamped to the final element,
+// which is then written twice with the same value), so all of them are in flight together.
+template <int RS, int RP>
+__device__ __forceinline__ void stage_region(float* dst, const float* in, const int* ry, const int* rx, int w) {
+  constexpr int kPer = (RS * RS + kThreads - 1) / kThreads;
+  float v[kPer];
+  int at[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int e = min((int)threadIdx.x + k * kThreads, RS * RS - 1);
+    const int ri = e / RS, rj = e - ri * RS;
+    at[k] = ri * RP + rj;
+    v[k] = in[(size_t)ext_pos(ry[ri]) * w + ext_pos(rx[rj])];
+  }
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) dst[at[k]] = v[k];
+}
+
+// 5-tap sum of v[0], v[step], ... v[4*step] in the order of an unmirrored position, or from v[4*step]
+// down to v[0] for a mirrored one: either way the chain adds the taps in the folded position's order.
+__device__ __forceinline__ float tap5(const float* v, int step, bool mirrored) {
+  float x[5];
+#pragma unroll
+  for (int b = 0; b < 5; ++b) x[b] = v[b * step];
+  float s = 0.f;
+#pragma unroll
+  for (int b = 0; b < 5; ++b) s = fmaf(c_h5[b], mirrored ? x[4 - b] : x[b], s);
+  return s;
+}
+__device__ __forceinline__ float tap9(const float* v, int step) {
+  float s = 0.f;
+#pragma unroll
+  for (int b = 0; b < 9; ++b) s = fmaf(c_h9[b], v[b * step], s);
+  return s;
+}
+
+// Scales s and s+1 of a decomposition: w_s, w_{s+1} and (OUTC) C_{s+2} from C_s.  The filters are
+// applied separably, row sums first (the one-scale kernel's order): pass 1 forms the 5-tap row sums
+// of C_s at every C_{s+1} column and the 9-tap row sums at the tile's columns, pass 2 the column sums
+// (C_{s+1} and w_s), pass 3 the row sums of C_{s+1} at spacing 2, pass 4 w_{s+1} and C_{s+2}.
+template <bool MASK, bool OUTC>
+__global__ __launch_bounds__(kThreads) void k_iuwt_small2_dec(Small2Params<Dec2Job> sp) {
+  constexpr int RS = kTile2 + 20, RP = RS + 1;  // staged C_s: lattice -10 .. 25 (rows padded)
+  constexpr int IS = kTile2 + 16, IP = IS + 1;  // C_{s+1}: lattice -8 .. 23
+  constexpr int N5 = RS * IS;                   // pass 1: 5-tap row sums, every staged row
+  constexpr int N9 = (kTile2 + 8) * kTile2;     // pass 1: 9-tap row sums, staged rows 6 .. 29
+  constexpr int M9 = IS * kTile2;               // pass 3: 9-tap row sums of C_{s+1}, every row
+  constexpr int M5 = (kTile2 + 8) * kTile2;     // pass 3: 5-tap row sums of C_{s+1}, rows 4 .. 27
+  static_assert((N5 + N9) % kThreads == 0 && N5 % 64 == 0, "pass 1 splits evenly, wave-uniform");
+  static_assert(IS * IS % kThreads == 0 && M9 % 64 == 0, "pass 2 splits evenly; pass 3 wave-uniform");
+  static_assert(M9 + M5 <= RS * IP, "pass 3 reuses the pass-1 array");
+  __shared__ float cs[RS * RP];
+  __shared__ float hs[RS * IP];  // pass 1 5-tap sums [row][column]; pass 3 sums afterwards
+  __shared__ float h9[N9];
+  __shared__ float c1[IS * IP];
+  __shared__ int ry[RS], rx[RS];
+  const Dec2Job a = sp.job[blockIdx.z];
+  if (a.pred && *a.pred == 0) return;
+  const int h = sp.h, w = sp.w, d = sp.d;
+  const int pr = blockIdx.y / sp.ntr, i0 = (blockIdx.y - pr * sp.ntr) * kTile2;
+  const int pc = blockIdx.x / sp.ntc, j0 = (blockIdx.x - pc * sp.ntc) * kTile2;
+  if (threadIdx.x < 64)
+    ext_indices(ry, RS, pr, d, i0 - 10, h);
+  else if (threadIdx.x < 128)
+    ext_indices(rx, RS, pc, d, j0 - 10, w);
+  const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
+  const int row = pr + d * (i0 + ty), col = pc + d * (j0 + tx);
+  const bool live = row < h && col < w;
+  const size_t off = live ? (size_t)row * w + col : 0;
+  const bool keep0 = !MASK || a.mask0[off] != 0;
+  const bool keep1 = !MASK || a.mask1[off] != 0;
+  __syncthreads();
+  stage_region<RS, RP>(cs, a.in, ry, rx, w);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < (N5 + N9) / kThreads; ++k) {
+    const int e = threadIdx.x + k * kThreads;
+    if (e < N5) {
+      const int ri = e / IS, v = e - ri * IS;
+      hs[ri * IP + v] = tap5(cs + ri * RP + v, 1, rx[v + 2] < 0);
+    } else {
+      const int f = e - N5, ri = f / kTile2, x = f - ri * kTile2;
+      h9[f] = tap9(cs + (ri + 6) * RP + x + 6, 1);
+    }
+  }
+  __syncthreads();
+  // C_{s+1} slot (u, v) is centred on staged (u + 2, v + 2); w_s on staged (ty + 10, tx + 10)
+#pragma unroll
+  for (int k = 0; k < IS * IS / kThreads; ++k) {
+    const int e = threadIdx.x + k * kThreads;
+    const int u = e / IS, v = e - u * IS;
+    c1[u * IP + v] = tap5(hs + u * IP + v, IP, ry[u + 2] < 0);
+  }
+  if (live) a.out_w0[off] = keep0 ? cs[(ty + 10) * RP + tx + 10] - tap9(h9 + ty * kTile2 + tx, kTile2) : 0.f;
+  __syncthreads();
+  float* g9 = hs;       // [32][16]: 9-tap sums at spacing 2 of C_{s+1} row r at tile column x
+  float* g5 = hs + M9;  // [24][16]: 5-tap sums of C_{s+1} row r + 4
+#pragma unroll
+  for (int k = 0; k < (M9 + M5 + kThreads - 1) / kThreads; ++k) {
+    const int e = threadIdx.x + k * kThreads;
+    if (e < M9) {
+      const int r = e / kTile2, x = e - r * kTile2;
+      g9[e] = tap9(c1 + r * IP + x, 2);
+    } else if (OUTC && e < M9 + M5) {
+      const int f = e - M9, r = f / kTile2, x = f - r * kTile2;
+      g5[f] = tap5(c1 + (r + 4) * IP + x + 4, 2, false);
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+  // w_{s+1} centred on slot (ty + 8, tx + 8), taps on slot rows ty + 2i; C_{s+2} on rows ty + 4 + 2k
+  a.out_w1[off] = keep1 ? c1[(ty + 8) * IP + tx + 8] - tap9(g9 + ty * kTile2 + tx, 2 * kTile2) : 0.f;
+  if (OUTC) a.out_c[off] = tap5(g5 + ty * kTile2 + tx, 2 * kTile2, false);
+}
+
+// Scales s+1 then s of a recomposition: R_s = h_s T + w_s with T = h_{s+1} R + w_{s+1}.  Pass 1 forms
+// the row sums of R at spacing 2 at every T column, pass 2 T, pass 3 the row sums of T at the tile's
+// columns, pass 4 R_s.
+template <bool MASK>
+__global__ __launch_bounds__(kThreads) void k_iuwt_small2_rec(Small2Params<PairJob> sp) {
+  constexpr int RS = kTile2 + 12, RP = RS + 1;  // staged R: lattice -6 .. 21
+  constexpr int TS = kTile2 + 4, TP = TS + 1;   // T: lattice -2 .. 17
+  constexpr int N1 = RS * TS;                   // pass 1 sums
+  constexpr int N2 = TS * TS;                   // T slots
+  constexpr int N3 = TS * kTile2;               // pass 3 sums
+  constexpr int K2 = (N2 + kThreads - 1) / kThreads;
+  __shared__ float rs[RS * RP];
+  __shared__ float hr[RS * TP];  // pass 1 sums [row][T column]; pass 3 sums afterwards
+  __shared__ float tt[TS * TP];
+  __shared__ int ry[RS], rx[RS];
+  const PairJob a = sp.job[blockIdx.z];
+  if (a.pred && *a.pred == 0) return;
+  const int h = sp.h, w = sp.w, d = sp.d;
+  const int pr = blockIdx.y / sp.ntr, i0 = (blockIdx.y - pr * sp.ntr) * kTile2;
+  const int pc = blockIdx.x / sp.ntc, j0 = (blockIdx.x - pc * sp.ntc) * kTile2;
+  if (threadIdx.x < 64)
+    ext_indices(ry, RS, pr, d, i0 - 6, h);
+  else if (threadIdx.x < 128)
+    ext_indices(rx, RS, pc, d, j0 - 6, w);
+  const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
+  const int row = pr + d * (i0 + ty), col = pc + d * (j0 + tx);
+  const bool live = row < h && col < w;
+  const size_t off = live ? (size_t)row * w + col : 0;
+  float wi0 = a.w_in0[off];
+  if (MASK && a.mask0[off] == 0) wi0 = 0.f;
+  __syncthreads();
+  // the detail term of T slot (u, v) (staged (u + 4, v + 4)) is read at the folded position, where
+  // the one-scale launch adds it; the last slots are clamped (a repeated slot writes the same value)
+  float wi1[K2];
+#pragma unroll
+  for (int k = 0; k < K2; ++k) {
+    const int e = min((int)threadIdx.x + k * kThreads, N2 - 1);
+    const int u = e / TS, v = e - u * TS;
+    const size_t o = (size_t)ext_pos(ry[u + 4]) * w + ext_pos(rx[v + 4]);
+    wi1[k] = a.w_in1[o];
+    if (MASK && a.mask1[o] == 0) wi1[k] = 0.f;
+  }
+  stage_region<RS, RP>(rs, a.in, ry, rx, w);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < (N1 + kThreads - 1) / kThreads; ++k) {
+    const int e = min((int)threadIdx.x + k * kThreads, N1 - 1);
+    const int ri = e / TS, v = e - ri * TS;
+    hr[ri * TP + v] = tap5(rs + ri * RP + v, 2, rx[v + 4] < 0);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K2; ++k) {
+    const int e = min((int)threadIdx.x + k * kThreads, N2 - 1);
+    const int u = e / TS, v = e - u * TS;
+    tt[u * TP + v] = tap5(hr + u * TP + v, 2 * TP, ry[u + 4] < 0) + wi1[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < (N3 + kThreads - 1) / kThreads; ++k) {
+    const int e = min((int)threadIdx.x + k * kThreads, N3 - 1);
+    const int r = e / kTile2, x = e - r * kTile2;
+    hr[e] = tap5(tt + r * TP + x, 1, false);
+  }
+  __syncthreads();
+  if (!live) return;
+  a.out[off] = tap5(hr + ty * kTile2 + tx, kTile2, false) + wi0;
+}
+
+template <class J>
+Small2Params<J> small2_params(int h, int w, int s) {
+  Small2Params<J> sp{};
+  sp.h = h;
+  sp.w = w;
+  sp.d = 1 << s;
+  sp.ntr = ceil_div(ceil_div(h, sp.d), kTile2);
+  sp.ntc = ceil_div(ceil_div(w, sp.d), kTile2);
+  return sp;
+}
+
+template <class J>
+dim3 small2_grid(const Small2Params<J>& sp, int nb) { return dim3(sp.d * sp.ntc, sp.d * sp.ntr, nb); }
+
+// jobs[0..count) share shape and scale pair; masked and unmasked jobs, and jobs that keep C_{s+2}
+// and those that do not, go to separate launches
+void small2_dec_batch(const Dec2Job* jobs, int count, int h, int w, int s, hipStream_t st) {
+  Small2Params<Dec2Job> sp = small2_params<Dec2Job>(h, w, s);
+  for (int kind = 0; kind < 4; ++kind) {
+    const bool masked = (kind & 1) != 0, keep_c = (kind & 2) != 0;
+    int nb = 0;
+    auto flush = [&]() {
+      if (nb == 0) return;
+      const dim3 g = small2_grid(sp, nb);
+      switch (kind) {
+        case 0: hipLaunchKernelGGL((k_iuwt_small2_dec<false, false>), g, dim3(kThreads), 0, st, sp); break;
+        case 1: hipLaunchKernelGGL((k_iuwt_small2_dec<true, false>), g, dim3(kThreads), 0, st, sp); break;
+        case 2: hipLaunchKernelGGL((k_iuwt_small2_dec<false, true>), g, dim3(kThreads), 0, st, sp); break;
+        default: hipLaunchKernelGGL((k_iuwt_small2_dec<true, true>), g, dim3(kThreads), 0, st, sp); break;
+      }
+      PMS_LAUNCH_CHECK();
+      nb = 0;
+    };
+    for (int k = 0; k < count; ++k) {
+      if ((jobs[k].mask0 != nullptr) != masked || (jobs[k].out_c != nullptr) != keep_c) continue;
+      PMS_REQUIRE(jobs[k].out_w0 && jobs[k].out_w1 && (!masked || jobs[k].mask1),
+                  "two-scale decomposition needs both detail planes (and both masks or none)");
+      sp.job[nb++] = jobs[k];
+      if (nb == kMaxBatch) flush();
+    }
+    flush();
+  }
+}
+
+void small2_rec_batch(const PairJob* jobs, int count, int h, int w, int s, hipStream_t st) {
+  Small2Params<PairJob> sp = small2_params<PairJob>(h, w, s);
+  for (int masked = 0; masked < 2; ++masked) {
+    int nb = 0;
+    auto flush = [&]() {
+      if (nb == 0) return;
+      const dim3 g = small2_grid(sp, nb);
+      if (masked)
+        hipLaunchKernelGGL(k_iuwt_small2_rec<true>, g, dim3(kThreads), 0, st, sp);
+      else
+        hipLaunchKernelGGL(k_iuwt_small2_rec<false>, g, dim3(kThreads), 0, st, sp);
+      PMS_LAUNCH_CHECK();
+      nb = 0;
+    };
+    for (int k = 0; k < count; ++k) {
+      if ((jobs[k].mask0 != nullptr) != (masked != 0)) continue;
+      PMS_REQUIRE(!masked || jobs[k].mask1, "two-scale recomposition needs both masks or none");
+      sp.job[nb++] = jobs[k];
+      if (nb == kMaxBatch) flush();
+    }
+    flush();
+  }
+}
+
 int walk_flags(const IuwtScaleCall& c) {
   if (c.mode == IUWT_DECOMP)
     return (c.out_c ? F_OUTC : 0) | (c.out_w ? F_OUTW : 0) | (c.mask ? F_MASK : 0) | (c.ref ? F_REF : 0);
@@ -1217,6 +1506,12 @@ void check_iuwt_scale(int h, int w, int scale) {
 }
 
 bool use_small(int h, int w) { return (long)h * w <= tuning().small_pixels; }
+
+// Scales (s, s+1) of a small plane run as one two-scale launch unless PMS_IUWT_SMALL2=0 (read per
+// call; the planes are identical either way); scale s+1 must be within the row-walk range.
+bool small2_applies(int h, int w, int s) {
+  return use_small(h, w) && !iuwt_over_long(h, w, s + 1) && env_int("PMS_IUWT_SMALL2", 1) != 0;
+}
 
 dim3 small_grid(int h, int w) { return dim3(ceil_div(w, 64), ceil_div(h, 4)); }
 
@@ -1360,7 +1655,8 @@ static void decompose_batch_impl(IuwtDecompJob* jobs, int count, int h, int w, h
       pairs.clear();
       pwho.clear();
       Plan pl;
-      const bool pair_plan = fuse == 1 && mode == IUWT_DECOMP && dec2_plan(h, w, s, &pl);
+      const bool small_pair = mode == IUWT_DECOMP && small2_applies(h, w, s);
+      const bool pair_plan = small_pair || (fuse == 1 && mode == IUWT_DECOMP && dec2_plan(h, w, s, &pl));
       for (int k = 0; k < count; ++k) {
         const IuwtDecompJob& j = jobs[k];
         float* ping = j.work + (size_t)state[k].slot * plane;
@@ -1402,7 +1698,10 @@ static void decompose_batch_impl(IuwtDecompJob* jobs, int count, int h, int w, h
         who.push_back(k);
       }
       if (!pairs.empty()) {
-        dec2_batch(pairs.data(), (int)pairs.size(), h, w, s, pl, st);
+        if (small_pair)
+          small2_dec_batch(pairs.data(), (int)pairs.size(), h, w, s, st);
+        else
+          dec2_batch(pairs.data(), (int)pairs.size(), h, w, s, pl, st);
         for (int k : pwho) {
           State& sk = state[k];
           sk.done = s + 2;
@@ -1495,7 +1794,7 @@ static void recompose_batch_impl(const IuwtRecompJob* jobs, int count, int h, in
         --s;
         continue;
       }
-      if (pair && s - 1 >= j.scale_adjust && recomp2_plan(h, w, s - 1, &pl)) {
+      if (pair && s - 1 >= j.scale_adjust && (small2_applies(h, w, s - 1) || recomp2_plan(h, w, s - 1, &pl))) {
         sk.ops.push_back({OP_PAIR, s - 1});
         s -= 2;
       } else {
@@ -1556,8 +1855,12 @@ static void recompose_batch_impl(const IuwtRecompJob* jobs, int count, int h, in
       }
       if (op0.kind == OP_PAIR) {
         Plan pl;
-        PMS_REQUIRE(recomp2_plan(h, w, op0.s, &pl), "two-scale recomposition plan changed");
-        recomp2_batch(pairs.data(), (int)pairs.size(), h, w, op0.s, pl, st);
+        if (small2_applies(h, w, op0.s)) {
+          small2_rec_batch(pairs.data(), (int)pairs.size(), h, w, op0.s, st);
+        } else {
+          PMS_REQUIRE(recomp2_plan(h, w, op0.s, &pl), "two-scale recomposition plan changed");
+          recomp2_batch(pairs.data(), (int)pairs.size(), h, w, op0.s, pl, st);
+        }
       } else {
         iuwt_scale_batch(calls.data(), (int)calls.size(), st);
       }

@@ -169,6 +169,52 @@ def test_moresane_two_scale_decomposition_is_exact(hip, monkeypatch):
     assert out[0].trace == out[1].trace
 
 
+@pytest.mark.parametrize("shape,scales,adjust", [
+    ((256, 256), 4, 0), ((256, 256), 8, 0), ((256, 256), 9, 1), ((255, 201), 5, 0), ((64, 64), 5, 1),
+    ((40, 40), 4, 0), ((128, 512), 6, 2), ((512, 128), 5, 0), ((17, 23), 3, 0), ((200, 31), 3, 0),
+])
+def test_iuwt_small_plane_scale_pairs_are_exact(hip, shape, scales, adjust, monkeypatch):
+    """Planes of at most 256^2 pixels take scale pairs in one launch (the intermediate plane only in
+    LDS, border positions at unfolded coordinates): decomposition and recomposition give the same
+    bits as one launch per scale, up to dilations of a quarter of the side and next to scales beyond
+    the side."""
+    say("iuwt small-plane pairs vs single scales %s S=%d adj=%d" % (shape, scales, adjust))
+    x = np.random.default_rng(11).standard_normal(shape).astype(np.float32)
+    out = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("PMS_IUWT_SMALL2", flag)
+        det, sm = iuwt.iuwt_decomposition(x, scales, adjust, store_smoothed=True)
+        det_only = iuwt.iuwt_decomposition(x, scales, adjust, store_smoothed=False)
+        rec = iuwt.iuwt_recomposition(det, adjust, smoothed_array=sm)
+        rec_d = iuwt.iuwt_recomposition(det, adjust)
+        out.append((det, sm, det_only, rec, rec_d))
+    for a, b in zip(out[0], out[1]):
+        np.testing.assert_array_equal(a, b)
+    rdet, rsm = oiuwt.iuwt_decomposition(x, scales, adjust, store_smoothed=True)
+    for k in range(rdet.shape[0]):
+        assert rel(out[0][0][k], rdet[k]) < 1e-5, (k, rel(out[0][0][k], rdet[k]))
+    assert rel(out[0][1], rsm) < 1e-6
+    assert rel(out[0][4], oiuwt.iuwt_recomposition(rdet, adjust, None)) < 1e-5
+
+
+def test_moresane_small_plane_scale_pairs_are_exact(hip, monkeypatch):
+    """Masked decompositions and recompositions of 256^2 planes inside the minor loop: the whole
+    deconvolution is bit-identical with and without two-scale launches."""
+    say("moresane 256^2, S=4, small-plane scale pairs on/off")
+    from pymoresane_amd.main import DataImage
+    dirty, psf = synth.make_pair(256, 33, n_points=30, n_blobs=3)
+    kw = dict(scale_count=4, loop_gain=0.2, major_loop_miter=4, minor_loop_miter=30, enforce_positivity=True)
+    out = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("PMS_IUWT_SMALL2", flag)
+        img = DataImage(dirty, psf, None, 1.0, 1.0)
+        img.moresane(**kw)
+        out.append(img)
+    np.testing.assert_array_equal(out[0].model, out[1].model)
+    np.testing.assert_array_equal(out[0].residual, out[1].residual)
+    assert out[0].trace == out[1].trace
+
+
 def test_iuwt_round_trip_at_full_size(hip):
     """4096^2, S=7, store_smoothed: decomposition then recomposition reproduces the input."""
     say("iuwt round trip 4096^2 S=7")
