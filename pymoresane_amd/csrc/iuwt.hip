@@ -1507,10 +1507,21 @@ void check_iuwt_scale(int h, int w, int scale) {
 
 bool use_small(int h, int w) { return (long)h * w <= tuning().small_pixels; }
 
-// Scales (s, s+1) of a small plane run as one two-scale launch unless PMS_IUWT_SMALL2=0 (read per
-// call; the planes are identical either way); scale s+1 must be within the row-walk range.
-bool small2_applies(int h, int w, int s) {
-  return use_small(h, w) && !iuwt_over_long(h, w, s + 1) && env_int("PMS_IUWT_SMALL2", 1) != 0;
+// Kernel family of a batched decomposition or recomposition of `jobs` planes: the direct-stencil
+// kernels for planes of at most 256^2 pixels, and for planes of at most 512^2 while the launch holds at
+// most PMS_IUWT_SMALL_LAUNCH pixels (default 2^22: 16 planes of 512^2; read per call).  Few mid-sized
+// planes leave the row walks with too few workgroups to fill the chip, while a large batch of them
+// streams better through the walks (whose planes are bit-identical).
+bool use_small_launch(int h, int w, int jobs) {
+  const long px = (long)h * w;
+  return use_small(h, w) || (px <= 512L * 512 && px * jobs <= (long)env_int("PMS_IUWT_SMALL_LAUNCH", 1 << 22));
+}
+
+// Scales (s, s+1) of a plane on the direct-stencil kernels run as one two-scale launch unless
+// PMS_IUWT_SMALL2=0 (read per call; the planes are identical either way); scale s+1 must be within
+// the row-walk range.
+bool small2_applies(bool small, int h, int w, int s) {
+  return small && !iuwt_over_long(h, w, s + 1) && env_int("PMS_IUWT_SMALL2", 1) != 0;
 }
 
 dim3 small_grid(int h, int w) { return dim3(ceil_div(w, 64), ceil_div(h, 4)); }
@@ -1528,10 +1539,12 @@ static ScaleJob scale_job(const IuwtScaleCall& c) {
   return j;
 }
 
-int iuwt_scale(const IuwtScaleCall& c, hipStream_t st) { return iuwt_scale_batch(&c, 1, st); }
+int iuwt_scale(const IuwtScaleCall& c, hipStream_t st) { return iuwt_scale_batch(&c, 1, st, use_small(c.h, c.w)); }
 
-// All calls share mode, shape and scale; returns the blocks (partials) per job.
-int iuwt_scale_batch(const IuwtScaleCall* calls, int count, hipStream_t st) {
+// All calls share mode, shape and scale; `small` picks the direct-stencil kernel over the row walk
+// (a caller passing a reference plane passes use_small(h, w), the rule iuwt_scale_blocks counts
+// partials by).  Returns the blocks (partials) per job.
+int iuwt_scale_batch(const IuwtScaleCall* calls, int count, hipStream_t st, bool small) {
   PMS_REQUIRE(count >= 1, "empty IUWT batch");
   const IuwtScaleCall& c0 = calls[0];
   check_iuwt_scale(c0.h, c0.w, c0.scale);
@@ -1539,7 +1552,6 @@ int iuwt_scale_batch(const IuwtScaleCall* calls, int count, hipStream_t st) {
   for (int k = 1; k < count; ++k)
     PMS_REQUIRE(calls[k].mode == c0.mode && calls[k].h == c0.h && calls[k].w == c0.w && calls[k].scale == c0.scale,
                 "IUWT batch mixes modes, shapes or scales");
-  const bool small = use_small(c0.h, c0.w);
   const int halo = (c0.mode == IUWT_DECOMP) ? 4 : 2;
   Plan pl{};
   if (!small) {
@@ -1620,7 +1632,7 @@ long iuwt_decompose(const float* in, int h, int w, int scale_count, int scale_ad
 
 // Scale by scale over all jobs: at scale s a job either pre-smooths (s < scale_adjust) or
 // decomposes (scale_adjust <= s < scale_count); jobs of the same kind share one launch.
-static void decompose_batch_impl(IuwtDecompJob* jobs, int count, int h, int w, hipStream_t st) {
+static void decompose_batch_impl(IuwtDecompJob* jobs, int count, int h, int w, bool small, hipStream_t st) {
   const size_t plane = (size_t)h * w;
   struct State {
     const float* cur;
@@ -1655,8 +1667,8 @@ static void decompose_batch_impl(IuwtDecompJob* jobs, int count, int h, int w, h
       pairs.clear();
       pwho.clear();
       Plan pl;
-      const bool small_pair = mode == IUWT_DECOMP && small2_applies(h, w, s);
-      const bool pair_plan = small_pair || (fuse == 1 && mode == IUWT_DECOMP && dec2_plan(h, w, s, &pl));
+      const bool small_pair = mode == IUWT_DECOMP && small2_applies(small, h, w, s);
+      const bool pair_plan = small_pair || (!small && fuse == 1 && mode == IUWT_DECOMP && dec2_plan(h, w, s, &pl));
       for (int k = 0; k < count; ++k) {
         const IuwtDecompJob& j = jobs[k];
         float* ping = j.work + (size_t)state[k].slot * plane;
@@ -1712,7 +1724,7 @@ static void decompose_batch_impl(IuwtDecompJob* jobs, int count, int h, int w, h
         }
       }
       if (calls.empty()) continue;
-      const int nb = iuwt_scale_batch(calls.data(), (int)calls.size(), st);
+      const int nb = iuwt_scale_batch(calls.data(), (int)calls.size(), st, small);
       for (size_t i = 0; i < who.size(); ++i) {
         IuwtDecompJob& j = jobs[who[i]];
         State& sk = state[who[i]];
@@ -1747,7 +1759,7 @@ void iuwt_recompose(const float* detail, int nplanes, int h, int w, int scale_ad
 // the shape allows it (the same planes result).  Each job's steps form a list; round r issues every
 // job's r-th step, steps of the same kind and scale sharing one launch.  Without a smoothed base R
 // starts at zero, so a job's coarsest step is its (masked) detail plane itself.
-static void recompose_batch_impl(const IuwtRecompJob* jobs, int count, int h, int w, hipStream_t st) {
+static void recompose_batch_impl(const IuwtRecompJob* jobs, int count, int h, int w, bool small, hipStream_t st) {
   const size_t plane = (size_t)h * w;
   enum { OP_RECOMP, OP_SMOOTH, OP_PAIR };
   struct Op {
@@ -1794,7 +1806,7 @@ static void recompose_batch_impl(const IuwtRecompJob* jobs, int count, int h, in
         --s;
         continue;
       }
-      if (pair && s - 1 >= j.scale_adjust && (small2_applies(h, w, s - 1) || recomp2_plan(h, w, s - 1, &pl))) {
+      if (pair && s - 1 >= j.scale_adjust && (small2_applies(small, h, w, s - 1) || (!small && recomp2_plan(h, w, s - 1, &pl)))) {
         sk.ops.push_back({OP_PAIR, s - 1});
         s -= 2;
       } else {
@@ -1855,14 +1867,14 @@ static void recompose_batch_impl(const IuwtRecompJob* jobs, int count, int h, in
       }
       if (op0.kind == OP_PAIR) {
         Plan pl;
-        if (small2_applies(h, w, op0.s)) {
+        if (small2_applies(small, h, w, op0.s)) {
           small2_rec_batch(pairs.data(), (int)pairs.size(), h, w, op0.s, st);
         } else {
           PMS_REQUIRE(recomp2_plan(h, w, op0.s, &pl), "two-scale recomposition plan changed");
           recomp2_batch(pairs.data(), (int)pairs.size(), h, w, op0.s, pl, st);
         }
       } else {
-        iuwt_scale_batch(calls.data(), (int)calls.size(), st);
+        iuwt_scale_batch(calls.data(), (int)calls.size(), st, small);
       }
     }
   }
@@ -2048,11 +2060,14 @@ void iuwt_decompose_batch(IuwtDecompJob* jobs, int count, int h, int w, hipStrea
       sig += " S" + std::to_string(j.scale_count) + "/" + std::to_string(j.scale_adjust) + (j.detail ? " w" : "") +
              (j.smoothed ? " c" : "") + (j.mask ? " m" : "");
   }
-  if (has_ref) {
-    decompose_batch_impl(jobs, count, h, w, st);
+  // per-block partials against a reference plane are counted by the plane-size rule alone
+  const bool small = has_ref ? use_small(h, w) : use_small_launch(h, w, count);
+  if (has_ref || small) {
+    decompose_batch_impl(jobs, count, h, w, small, st);
     return;
   }
-  autotuned(sig, walk_candidates(h, w, false), can_time, st, [&]() { decompose_batch_impl(jobs, count, h, w, st); });
+  autotuned(sig, walk_candidates(h, w, false), can_time, st,
+            [&]() { decompose_batch_impl(jobs, count, h, w, false, st); });
 }
 
 void iuwt_recompose_batch(const IuwtRecompJob* jobs, int count, int h, int w, hipStream_t st) {
@@ -2071,7 +2086,12 @@ void iuwt_recompose_batch(const IuwtRecompJob* jobs, int count, int h, int w, hi
       sig += " S" + std::to_string(j.nplanes) + "/" + std::to_string(j.scale_adjust) + (j.smoothed ? " c" : "") +
              (j.mask ? " m" : "");
   }
-  autotuned(sig, walk_candidates(h, w, true), can_time, st, [&]() { recompose_batch_impl(jobs, count, h, w, st); });
+  if (use_small_launch(h, w, count)) {
+    recompose_batch_impl(jobs, count, h, w, true, st);
+    return;
+  }
+  autotuned(sig, walk_candidates(h, w, true), can_time, st,
+            [&]() { recompose_batch_impl(jobs, count, h, w, false, st); });
 }
 
 }  // namespace pms

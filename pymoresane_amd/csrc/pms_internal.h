@@ -146,8 +146,9 @@ long iuwt_decompose_partials(int h, int w, int scale_count, int scale_adjust);
 void iuwt_recompose(const float* detail, int nplanes, int h, int w, int scale_adjust,
                     const float* smoothed, float* out, float* work, const uint8_t* mask,
                     const int* pred, hipStream_t st);
-// calls of one mode, shape and scale for several jobs, in launches of up to kMaxBatch jobs
-int iuwt_scale_batch(const IuwtScaleCall* calls, int count, hipStream_t st);
+// calls of one mode, shape and scale for several jobs, in launches of up to kMaxBatch jobs, on the
+// direct-stencil kernel (small) or the row walk
+int iuwt_scale_batch(const IuwtScaleCall* calls, int count, hipStream_t st, bool small);
 // iuwt_decompose / iuwt_recompose for several jobs of one plane shape; every job keeps its own
 // scale range, buffers, mask and predicate.  Decomposition jobs report their partial count in np.
 template <class T>

@@ -139,6 +139,7 @@ def test_iuwt_two_scale_decomposition_is_exact(hip, shape, scales, adjust, monke
     say("iuwt decomposition pairs vs single scales %s S=%d adj=%d" % (shape, scales, adjust))
     x = np.random.default_rng(9).standard_normal(shape).astype(np.float32)
     monkeypatch.setenv("PMS_AUTOTUNE", "0")
+    monkeypatch.setenv("PMS_IUWT_SMALL_LAUNCH", "0")  # row walks for every plane above 256^2
     monkeypatch.setenv("PMS_IUWT_DEC2", "1")
     det_f, sm_f = iuwt.iuwt_decomposition(x, scales, adjust, store_smoothed=True)
     monkeypatch.setenv("PMS_IUWT_DEC2", "0")
@@ -213,6 +214,36 @@ def test_moresane_small_plane_scale_pairs_are_exact(hip, monkeypatch):
     np.testing.assert_array_equal(out[0].model, out[1].model)
     np.testing.assert_array_equal(out[0].residual, out[1].residual)
     assert out[0].trace == out[1].trace
+
+
+def test_iuwt_mid_planes_same_on_both_kernel_families(hip, monkeypatch):
+    """Planes of up to 512^2 pixels take the direct-stencil kernels when a launch holds few of them and
+    the row walks otherwise (PMS_IUWT_SMALL_LAUNCH=0 forces the walks): both give the same bits, for
+    decompositions, recompositions and a whole masked deconvolution."""
+    from pymoresane_amd.main import DataImage
+    dirty, psf = synth.make_pair(512, 41, n_points=30, n_blobs=3)
+    out = []
+    for flag in (None, "0"):
+        if flag is None:
+            monkeypatch.delenv("PMS_IUWT_SMALL_LAUNCH", raising=False)
+        else:
+            monkeypatch.setenv("PMS_IUWT_SMALL_LAUNCH", flag)
+        res = []
+        for shape, scales, adjust in [((512, 512), 6, 0), ((384, 512), 5, 1), ((300, 500), 5, 0)]:
+            say("iuwt %s S=%d adj=%d, kernel family %s" % (shape, scales, adjust, flag or "by launch size"))
+            x = np.random.default_rng(3).standard_normal(shape).astype(np.float32)
+            det, sm = iuwt.iuwt_decomposition(x, scales, adjust, store_smoothed=True)
+            res += [det, sm, iuwt.iuwt_recomposition(det, adjust, smoothed_array=sm), iuwt.iuwt_recomposition(det, adjust)]
+        say("moresane 512^2 S=5, kernel family %s" % (flag or "by launch size"))
+        img = DataImage(dirty, psf, None, 1.0, 1.0)
+        img.moresane(scale_count=5, loop_gain=0.2, major_loop_miter=3, minor_loop_miter=30, enforce_positivity=True)
+        res += [img.model, img.residual, repr(img.trace)]
+        out.append(res)
+    for a, b in zip(out[0], out[1]):
+        if isinstance(a, str):
+            assert a == b
+        else:
+            np.testing.assert_array_equal(a, b)
 
 
 def test_iuwt_round_trip_at_full_size(hip):
@@ -526,7 +557,7 @@ def run_pair(dirty, psf, **kw):
     return img, ref
 
 
-@pytest.mark.parametrize("case", ["c0", "walk"])
+@pytest.mark.parametrize("case", ["c0", "mid", "walk"])
 def test_moresane_incremental_snr_matches_direct(hip, case, monkeypatch):
     """The minor loop's SNR sums by linearity (M.W(H x) carried across iterations, updated with alpha
     times this iteration's M.W(H p)) against the direct convolution + decomposition of xn that
@@ -538,7 +569,9 @@ def test_moresane_incremental_snr_matches_direct(hip, case, monkeypatch):
         kw = dict(scale_count=4, loop_gain=0.2, sigma_level=4, major_loop_miter=30, tolerance=0.75,
                   enforce_positivity=True, conv_mode="linear", minor_loop_miter=30, accuracy=1e-6)
     else:
-        dirty, psf = synth.make_pair(1024, 33, n_points=30, n_blobs=3)
+        # "mid": 512^2 planes, whose reference-plane decompositions take the row walks while the
+        # others take the direct-stencil kernels
+        dirty, psf = synth.make_pair(1024 if case == "walk" else 512, 33, n_points=30, n_blobs=3)
         kw = dict(scale_count=4, loop_gain=0.2, major_loop_miter=3, minor_loop_miter=12, enforce_positivity=True)
     say("moresane %s: incremental vs direct SNR" % case)
     out = []
