@@ -446,8 +446,9 @@ __device__ __forceinline__ void ccl_union(int* lab, int a, int b) {
   }
 }
 
-// Two-level labelling: each 32x32 tile is labelled in LDS (union-find with LDS atomicMin links, so
-// every tile component's root is its smallest raster index), the tile roots are written as global
+// Two-level labelling: each 32x32 tile is labelled in LDS (row runs linked to their first pixel,
+// then union-find with LDS atomicMin links between rows, so every tile component's root is its
+// smallest raster index), the tile roots are written as global
 // raster indices, then only pixels on tile borders union across tiles in global memory.  The final
 // forest, and so the roots, are those of a global union-find: every component's root is its
 // smallest raster index.
@@ -490,20 +491,32 @@ __global__ __launch_bounds__(kT) void k_ccl_tile(const V* __restrict__ w, int* _
   const size_t plane = (size_t)h * wd;
   const V* x = w + (size_t)s * plane;
   int* l = lab + (size_t)s * plane;
+  // A tile row is half a wave: one ballot gives each pixel the unset pixels left of it in its row, so
+  // every run of set pixels starts as a tree one link deep under its first pixel (its smallest index)
+  // and row neighbours need no union.
+  static_assert(T == 32 && TT % kT == 0 && kT % 64 == 0, "one tile row per half wave, uniform loop");
   for (int i = threadIdx.x; i < TT; i += kT) {
     const int r = r0 + i / T, c = c0 + i % T;
-    L[i] = (r < h && c < wd && x[(size_t)r * wd + c] != (V)0) ? i : -1;
+    const bool on = r < h && c < wd && x[(size_t)r * wd + c] != (V)0;
+    const unsigned long long m = __ballot(on);
+    const int lo = i & 31;  // == this lane's position within its half wave
+    const unsigned gaps = ~(unsigned)(m >> (threadIdx.x & 32)) & ((1u << lo) - 1u);
+    L[i] = on ? i - lo + (gaps ? 32 - __clz(gaps) : 0) : -1;
   }
   __syncthreads();
+  // 8-connected unions with the row above: the pixel above alone when it is set (the diagonal ones
+  // are then in its run), otherwise each diagonal pixel that is set
   for (int i = threadIdx.x; i < TT; i += kT) {
     if (L[i] < 0) continue;
     const int tr = i / T, tc = i % T;
-    if (tc > 0 && L[i - 1] >= 0) lds_union(L, i, i - 1);
     if (tr > 0) {
       const int up = i - T;
-      if (tc > 0 && L[up - 1] >= 0) lds_union(L, i, up - 1);
-      if (L[up] >= 0) lds_union(L, i, up);
-      if (tc + 1 < T && L[up + 1] >= 0) lds_union(L, i, up + 1);
+      if (L[up] >= 0) {
+        lds_union(L, i, up);
+      } else {
+        if (tc > 0 && L[up - 1] >= 0) lds_union(L, i, up - 1);
+        if (tc + 1 < T && L[up + 1] >= 0) lds_union(L, i, up + 1);
+      }
     }
   }
   __syncthreads();
