@@ -28,9 +28,10 @@
 // A recomposition handles two consecutive detail scales per launch where the
 // shape allows (k_iuwt_recomp2: the coarser scale is the same filter at
 // spacing 2 in the finer scale's frame), with results bit-identical to one
-// launch per scale.  Planes of at most 256^2 pixels use a direct-stencil
-// kernel instead of the walks, and pairs of scales there run as one launch
-// (k_iuwt_small2_dec / k_iuwt_small2_rec, also bit-identical).  An autotuner (end of this file) picks the
+// launch per scale.  Planes of at most 256^2 pixels, and planes of at most
+// 512^2 in launches of few of them, use direct-stencil kernels instead of the
+// walks, and pairs of scales there run as one launch (k_iuwt_small2_dec /
+// k_iuwt_small2_rec); every family gives the same bits.  An autotuner (end of this file) picks the
 // walks' workgroup count and the recomposition pairing per call signature.
 #include <algorithm>
 #include <cmath>
@@ -887,7 +888,7 @@ int env_int(const char* name, int dflt) {
 // tuning knobs, read once per process
 struct IuwtTuning {
   int qmax, krows, target_blocks;
-  long small_pixels;  // planes up to this many pixels use the direct-stencil kernel
+  long small_pixels;  // planes up to this many pixels always use the direct-stencil kernels
   int pair_min_rows;  // two-scale recomposition needs this many rows per row phase of the finer scale
   IuwtTuning()
       : qmax(env_int("PMS_IUWT_QMAX", 32)),
